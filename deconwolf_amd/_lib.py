@@ -1,0 +1,105 @@
+"""ctypes binding of libdwhip.so (include/dwhip.h).
+
+The library is built in-tree by deconwolf_amd.build.  There is no fallback:
+if the library is missing or cannot be loaded, every entry point raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdwhip.so")
+
+DWHIP_OK = 0
+ERRORS = {
+    -1: "DWHIP_ERR_ARG", -2: "DWHIP_ERR_SIZE", -3: "DWHIP_ERR_HIP", -4: "DWHIP_ERR_NOMEM",
+    -5: "DWHIP_ERR_PSF", -6: "DWHIP_ERR_NODEVICE", -7: "DWHIP_ERR_STATE",
+}
+
+ITER_ABS, ITER_REL, ITER_FIXED = 0, 1, 2
+METRIC_MSE, METRIC_IDIV = 0, 1
+START_FLAT, START_LP, START_IDENTITY = 0, 1, 2
+
+
+class DwhipParams(ctypes.Structure):
+    _fields_ = [
+        ("niter", ctypes.c_int),
+        ("iter_type", ctypes.c_int),
+        ("maxiter", ctypes.c_int),
+        ("err_rel", ctypes.c_float),
+        ("err_abs", ctypes.c_float),
+        ("bg", ctypes.c_float),
+        ("bg_auto", ctypes.c_int),
+        ("border_quality", ctypes.c_int),
+        ("positivity", ctypes.c_int),
+        ("alphamax", ctypes.c_float),
+        ("metric", ctypes.c_int),
+        ("start", ctypes.c_int),
+        ("start_image", ctypes.c_void_p),
+        ("work", ctypes.c_int64 * 3),
+        ("verbosity", ctypes.c_int),
+    ]
+
+
+class DwhipStats(ctypes.Structure):
+    _fields_ = [
+        ("bg", ctypes.c_float),
+        ("iterations", ctypes.c_int),
+        ("work", ctypes.c_int64 * 3),
+        ("ref_work", ctypes.c_int64 * 3),
+        ("setup_ms", ctypes.c_double),
+        ("iter_ms", ctypes.c_double),
+        ("err_trace", ctypes.c_void_p),
+        ("err_trace_len", ctypes.c_int),
+    ]
+
+
+class DwhipError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = ERRORS.get(code, str(code))
+        try:
+            msg += ": " + lib().dwhip_strerror(code).decode()
+        except Exception:  # pragma: no cover - message decoration only
+            pass
+        super().__init__((what + ": " if what else "") + msg)
+
+
+# (name, restype, argtypes) of every symbol include/dwhip.h declares
+_vp, _i64, _int = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+_P64 = ctypes.POINTER(ctypes.c_int64)
+SIGNATURES = [
+    ("dwhip_params_default", None, [ctypes.POINTER(DwhipParams), _int]),
+    ("dwhip_strerror", ctypes.c_char_p, [_int]),
+    ("dwhip_init", _int, []),
+    ("dwhip_work_size", _int, [_i64, _i64, _i64, _i64, _i64, _i64, _int, _P64, _P64]),
+    ("dwhip_job_bytes", ctypes.c_size_t, [_P64, _i64, _i64, _i64, _i64, _i64, _i64, _int, _int]),
+    ("dwhip_shb_run", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                             ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats)]),
+    ("dwhip_shb_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                                    ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
+    ("dwhip_fft3_device", _int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libdwhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libdwhip.so is not built (expected at %s); run `python -m deconwolf_amd.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code, what=""):
+    if code != DWHIP_OK:
+        raise DwhipError(code, what)
+    return code

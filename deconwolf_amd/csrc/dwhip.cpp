@@ -1,0 +1,664 @@
+// Host side of libdwhip: transform plans, job set-up and the per-image SHB
+// driver behind the C-ABI in include/dwhip.h.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "dwhip.h"
+#include "dwhip_internal.h"
+#include "shb_kernels.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+const char* dwhip_strerror(int code)
+{
+    switch (code) {
+    case DWHIP_OK: return "ok";
+    case DWHIP_ERR_ARG: return "invalid argument";
+    case DWHIP_ERR_SIZE: return "transform size not supported (needs factors 2,3,5,7 and an even x extent)";
+    case DWHIP_ERR_HIP: return "HIP runtime error";
+    case DWHIP_ERR_NOMEM: return "device memory allocation failed";
+    case DWHIP_ERR_PSF: return "the PSF maximum could not be moved to the origin";
+    case DWHIP_ERR_NODEVICE: return "no usable GPU device";
+    case DWHIP_ERR_STATE: return "call not valid in this state";
+    default: return "unknown error";
+    }
+}
+
+int dwhip_hip_fail(hipError_t e, const char* what, int line)
+{
+    fprintf(stderr, "dwhip: %s failed (%s) at dwhip.cpp:%d\n", what, hipGetErrorString(e), line);
+    return e == hipErrorOutOfMemory ? DWHIP_ERR_NOMEM : DWHIP_ERR_HIP;
+}
+
+static int g_init = 0;
+
+int dwhip_init(void)
+{
+    if (g_init == 1) return DWHIP_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return DWHIP_ERR_NODEVICE;
+    if (hipSetDevice(0) != hipSuccess) return DWHIP_ERR_NODEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return DWHIP_ERR_NODEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        fprintf(stderr, "dwhip: device is %s, this build targets gfx950\n", prop.gcnArchName);
+        return DWHIP_ERR_NODEVICE;
+    }
+    g_init = 1;
+    return DWHIP_OK;
+}
+
+void dwhip_params_default(dwhip_params* p, int niter)
+{
+    memset(p, 0, sizeof(*p));
+    // dw_opts_new, src/dw.c:340-413
+    p->niter = niter;
+    p->iter_type = DWHIP_ITER_FIXED;
+    p->maxiter = 250;
+    p->err_rel = 0.02f;
+    p->err_abs = 1.0f;
+    p->bg = 1e-2f;
+    p->bg_auto = 1;
+    p->border_quality = 2;
+    p->positivity = 1;
+    p->alphamax = 1.0f;
+    p->metric = DWHIP_METRIC_IDIV;
+    p->start = DWHIP_START_FLAT;
+}
+
+// ---------------------------------------------------------------------------
+// sizes and plans
+// ---------------------------------------------------------------------------
+static bool is_smooth(int64_t n)
+{
+    if (n < 1) return false;
+    for (int f : {2, 3, 5, 7})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+static int64_t next_smooth(int64_t n)
+{
+    while (!is_smooth(n)) ++n;
+    return n;
+}
+
+// even nx with nx/2 smooth
+static int64_t next_smooth_even(int64_t n)
+{
+    int64_t h = (n + 1) / 2;
+    while (!is_smooth(h)) ++h;
+    return 2 * h;
+}
+
+static void ref_work_size(int bq, int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int64_t pP,
+                          int64_t* w)
+{
+    // method_shb.c:137-153
+    w[0] = M + pM - 1;
+    w[1] = N + pN - 1;
+    w[2] = P + pP - 1;
+    if (bq == 1) {
+        w[0] = M + (pM + 1) / 2;
+        w[1] = N + (pN + 1) / 2;
+        w[2] = P + (pP + 1) / 2;
+    }
+    if (bq == 0) {
+        w[0] = std::max(M, pM);
+        w[1] = std::max(N, pN);
+        w[2] = std::max(P, pP);
+    }
+}
+
+static bool size_ok(const int64_t* w)
+{
+    if (w[0] % 2 != 0 || !is_smooth(w[0] / 2) || !is_smooth(w[1]) || !is_smooth(w[2])) return false;
+    // kernel limits: at most DWHIP_XMAXE*NTHR complex per x-row tile,
+    // DWHIP_CMAXE*NTHR per column tile, 2^20 for the index arithmetic
+    if (w[0] / 2 > DWHIP_XMAXE * DWHIP_NTHR) return false;
+    if (w[1] > DWHIP_CMAXE * DWHIP_NTHR || w[2] > DWHIP_CMAXE * DWHIP_NTHR) return false;
+    if (w[0] < 2 || w[1] < 1 || w[2] < 1) return false;
+    return true;
+}
+
+int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int64_t pP,
+                    int border_quality, int64_t* work, int64_t* ref_work)
+{
+    if (M < 1 || N < 1 || P < 1 || pM < 1 || pN < 1 || pP < 1 || border_quality < 0 || border_quality > 2)
+        return DWHIP_ERR_ARG;
+    int64_t r[3];
+    ref_work_size(border_quality, M, N, P, pM, pN, pP, r);
+    if (ref_work)
+        for (int d = 0; d < 3; ++d) ref_work[d] = r[d];
+    int64_t w[3];
+    if (border_quality == 2) {
+        w[0] = next_smooth_even(r[0]);
+        w[1] = next_smooth(r[1]);
+        w[2] = next_smooth(r[2]);
+    } else {
+        for (int d = 0; d < 3; ++d) w[d] = r[d];
+    }
+    if (work)
+        for (int d = 0; d < 3; ++d) work[d] = w[d];
+    return size_ok(w) ? DWHIP_OK : DWHIP_ERR_SIZE;
+}
+
+// Radices of a 7-smooth n: 8s, then 4 or 2, then 5s, 3s, 7s.
+static std::vector<int> radices(int n)
+{
+    std::vector<int> r;
+    int m = n;
+    while (m % 8 == 0) { r.push_back(8); m /= 8; }
+    if (m % 4 == 0) { r.push_back(4); m /= 4; }
+    if (m % 2 == 0) { r.push_back(2); m /= 2; }
+    while (m % 5 == 0) { r.push_back(5); m /= 5; }
+    while (m % 3 == 0) { r.push_back(3); m /= 3; }
+    while (m % 7 == 0) { r.push_back(7); m /= 7; }
+    return r;
+}
+
+// Appends the plan's twiddles (forward sign) to tw, offsets relative to tw's start.
+int dwhip_make_plan(int n, FftPlanDev* p, std::vector<float2>& tw)
+{
+    memset(p, 0, sizeof(*p));
+    p->n = n;
+    std::vector<int> r = radices(n);
+    if ((int)r.size() > DWHIP_MAX_STAGES) return DWHIP_ERR_SIZE;
+    p->nst = (int)r.size();
+    int ns = 1;
+    for (int s = 0; s < p->nst; ++s) {
+        const int R = r[s];
+        p->radix[s] = R;
+        p->ns[s] = ns;
+        p->nsmagic[s] = fdiv_magic((uint32_t)ns);
+        p->twoff[s] = (int)tw.size();
+        for (int k = 0; k < ns; ++k)
+            for (int q = 1; q < R; ++q) {
+                const double a = -2.0 * M_PI * (double)(q * k) / (double)(ns * R);
+                tw.push_back(make_float2((float)cos(a), (float)sin(a)));
+            }
+        ns *= R;
+    }
+    return DWHIP_OK;
+}
+
+static int pow2_floor(int v)
+{
+    int p = 1;
+    while (p * 2 <= v) p *= 2;
+    return p;
+}
+
+// Tile shapes and LDS bytes for a transform size.
+void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
+{
+    const int h = (int)(w[0] / 2), ny = (int)w[1], nz = (int)w[2];
+    const int hc = h + 1;
+    t->xrows = std::max(1, std::min(8, DWHIP_XMAXE * DWHIP_NTHR / h));
+    t->ycols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_NTHR / ny)));
+    t->zcols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_NTHR / nz)));
+    t->ntile[KIND_X] = (ny * nz + t->xrows - 1) / t->xrows;
+    t->ntile[KIND_Y] = ((hc + t->ycols - 1) / t->ycols) * nz;
+    t->ntile[KIND_Z] = ((hc + t->zcols - 1) / t->zcols) * ny;
+    t->lds[KIND_X] = (size_t)t->xrows * h * sizeof(float2);
+    t->lds[KIND_Y] = (size_t)t->ycols * ny * sizeof(float2);
+    t->lds[KIND_Z] = (size_t)t->zcols * nz * sizeof(float2);
+}
+
+static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+size_t dwhip_job_bytes(const int64_t* work, int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN,
+                       int64_t pP, int border_quality, int niter)
+{
+    const size_t h = work[0] / 2, hcs = round_up(h + 1, 8);
+    const size_t V = (size_t)work[0] * work[1] * work[2];
+    const size_t Vc = hcs * work[1] * work[2];
+    DwhipTiles t;
+    dwhip_tile_config(work, &t);
+    size_t b = 2 * Vc * sizeof(float2);                  // C, K
+    b += (border_quality > 0 ? 3 : 2) * V * sizeof(float); // U0, U1, W
+    b += (size_t)t.ntile[KIND_X] * sizeof(double);
+    b += (size_t)(niter + 1) * (sizeof(float) + sizeof(double));
+    b += DWHIP_STAT_BLOCKS * 4 * sizeof(double) + 4096;
+    (void)M; (void)N; (void)P; (void)pM; (void)pN; (void)pP;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// jobs
+// ---------------------------------------------------------------------------
+#define HIPCHK(call)                                                   \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return dwhip_hip_fail(e_, #call, __LINE__); \
+    } while (0)
+
+void dwhip_job_free(DwhipJob* j)
+{
+    if (!j) return;
+    if (j->mem) hipFree(j->mem);
+    j->mem = nullptr;
+}
+
+// Builds the job (device buffers, plans) for device-resident im/psf.
+int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P,
+                     const float* psf_dev, int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm,
+                     const float* start_dev)
+{
+    memset(j, 0, sizeof(*j));
+    int64_t ref[3], w[3];
+    int rc = dwhip_work_size(M, N, P, pM, pN, pP, prm->border_quality, w, ref);
+    if (rc == DWHIP_ERR_ARG) return rc;
+    bool user_work = prm->work[0] > 0 || prm->work[1] > 0 || prm->work[2] > 0;
+    if (user_work) {
+        for (int d = 0; d < 3; ++d) w[d] = prm->work[d] > 0 ? prm->work[d] : w[d];
+        for (int d = 0; d < 3; ++d)
+            if (w[d] < ref[d]) return DWHIP_ERR_ARG;
+        if (prm->border_quality < 2)
+            for (int d = 0; d < 3; ++d)
+                if (w[d] != ref[d]) return DWHIP_ERR_ARG;  // padding changes bq 0/1 results
+        if (!size_ok(w)) return DWHIP_ERR_SIZE;
+    } else if (rc != DWHIP_OK) {
+        return rc;
+    }
+    if (M * N * P > (int64_t)1 << 31 || w[0] * w[1] * w[2] > (int64_t)1 << 34) return DWHIP_ERR_SIZE;
+    for (int d = 0; d < 3; ++d) {
+        j->work[d] = w[d];
+        j->ref_work[d] = ref[d];
+    }
+    const int niter = std::max(1, prm->iter_type == DWHIP_ITER_FIXED ? prm->niter : prm->maxiter);
+
+    JobDev& d = j->d;
+    d.nx = (int)w[0];
+    d.ny = (int)w[1];
+    d.nz = (int)w[2];
+    d.h = d.nx / 2;
+    d.hc = d.h + 1;
+    d.hcs = (int)round_up(d.hc, 8);
+    d.M = (int)M; d.N = (int)N; d.P = (int)P;
+    d.pM = (int)pM; d.pN = (int)pN; d.pP = (int)pP;
+    d.refMNP = (double)ref[0] * (double)ref[1] * (double)ref[2];
+    dwhip_tile_config(w, &j->tiles);
+    for (int k = 0; k < KIND_N; ++k) d.ntile[k] = j->tiles.ntile[k];
+    d.xrows = j->tiles.xrows;
+    d.ycols = j->tiles.ycols;
+    d.zcols = j->tiles.zcols;
+    d.positivity = prm->positivity;
+    d.metric = prm->metric;
+    d.has_w = prm->border_quality > 0;
+    d.bg_auto = prm->bg_auto;
+    d.start_identity = prm->start != DWHIP_START_FLAT;
+    d.niter = niter;
+    d.alphamax = prm->alphamax;
+    d.bg_user = prm->bg;
+    d.im = im_dev;
+    d.psf = psf_dev;
+    d.start_img = start_dev;
+
+    // twiddles
+    std::vector<float2> tw;
+    if (dwhip_make_plan(d.h, &d.px, tw) || dwhip_make_plan(d.ny, &d.py, tw) || dwhip_make_plan(d.nz, &d.pz, tw))
+        return DWHIP_ERR_SIZE;
+    const size_t rtw_off = tw.size();
+    for (int k = 0; k < d.h; ++k) {
+        const double a = -2.0 * M_PI * (double)k / (double)d.nx;
+        tw.push_back(make_float2((float)cos(a), (float)sin(a)));
+    }
+
+    // one allocation, carved into aligned pieces
+    const size_t V = (size_t)d.nx * d.ny * d.nz;
+    const size_t Vc = (size_t)d.hcs * d.ny * d.nz;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off = round_up(off + bytes, 256); return o; };
+    const size_t oC = take(Vc * sizeof(float2));
+    const size_t oK = take(Vc * sizeof(float2));
+    const size_t oU0 = take(V * sizeof(float));
+    const size_t oU1 = take(V * sizeof(float));
+    const size_t oW = d.has_w ? take(V * sizeof(float)) : 0;
+    const size_t oPart = take((size_t)d.ntile[KIND_X] * sizeof(double));
+    const size_t oStat = take(DWHIP_STAT_BLOCKS * 4 * sizeof(double));
+    const size_t oErr = take((size_t)niter * sizeof(float));
+    const size_t oErrd = take((size_t)niter * sizeof(double));
+    const size_t oTw = take(tw.size() * sizeof(float2));
+    const size_t oSt = take(sizeof(JobState));
+    const size_t oTab = take(sizeof(BatchDev) + 64 + sizeof(JobDev));
+    j->bytes = off;
+    hipError_t e = hipMalloc(&j->mem, off);
+    if (e != hipSuccess) {
+        j->mem = nullptr;
+        return e == hipErrorOutOfMemory ? DWHIP_ERR_NOMEM : DWHIP_ERR_HIP;
+    }
+    char* base = (char*)j->mem;
+    d.C = (float2*)(base + oC);
+    d.K = (float2*)(base + oK);
+    d.U0 = (float*)(base + oU0);
+    d.U1 = (float*)(base + oU1);
+    d.W = d.has_w ? (float*)(base + oW) : nullptr;
+    d.part = (double*)(base + oPart);
+    d.stat = (double*)(base + oStat);
+    d.err = (float*)(base + oErr);
+    d.errd = (double*)(base + oErrd);
+    d.tw = (const float2*)(base + oTw);
+    d.rtw = d.tw + rtw_off;
+    d.st = (JobState*)(base + oSt);
+    j->bd = (BatchDev*)(base + oTab);
+    j->jd = (JobDev*)(base + oTab + 64);
+
+    HIPCHK(hipMemcpy(base + oTw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(base + oC, 0, 2 * Vc * sizeof(float2)));
+    HIPCHK(hipMemset(d.part, 0, (size_t)d.ntile[KIND_X] * sizeof(double)));
+    HIPCHK(hipMemset(d.st, 0, sizeof(JobState)));
+    BatchDev bh;
+    memset(&bh, 0, sizeof(bh));
+    bh.njobs = 1;
+    for (int k = 0; k < KIND_N; ++k) {
+        d.tile0[k] = 0;
+        bh.ntile[k] = d.ntile[k];
+    }
+    HIPCHK(hipMemcpy(j->bd, &bh, sizeof(bh), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(j->jd, &d, sizeof(d), hipMemcpyHostToDevice));
+    return DWHIP_OK;
+}
+
+int dwhip_pass_x(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* jd, int ntile, size_t lds,
+                 hipStream_t s)
+{
+    if (ntile <= 0) return DWHIP_OK;
+    hipError_t e = dwhip_launch_xpass(mode, bd, jd, ntile, lds, s);
+    return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "x pass", __LINE__);
+    (void)j;
+}
+
+int dwhip_pass_c(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* jd, int ntile, size_t lds,
+                 hipStream_t s)
+{
+    if (ntile <= 0) return DWHIP_OK;
+    hipError_t e = dwhip_launch_colpass(mode, bd, jd, ntile, lds, s);
+    return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "column pass", __LINE__);
+    (void)j;
+}
+
+#define RCHK(call)                      \
+    do {                                \
+        int r_ = (call);                \
+        if (r_ != DWHIP_OK) return r_;  \
+    } while (0)
+
+// OTF, Bertero weights and start guess for a single job (method_shb.c:108-294).
+int dwhip_job_setup(const DwhipJob* j, hipStream_t s)
+{
+    const DwhipTiles& t = j->tiles;
+    const int nblk = DWHIP_STAT_BLOCKS;
+    HIPCHK(dwhip_launch_stats(j->jd, 0, nblk, s));
+    HIPCHK(dwhip_launch_start(j->jd, 0, 1024, s));
+    // OTF: shifted PSF -> R2C -> K / (nx*ny*nz)
+    RCHK(dwhip_pass_x(XM_PSF, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
+    RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+    RCHK(dwhip_pass_c(CM_ZOTF, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
+    if (j->d.has_w) {
+        // P1 = IFFT(conj(K) FFT(1_box)); W = P1 > 0.01 ? 1/P1 : 0
+        RCHK(dwhip_pass_x(XM_ONES, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
+        RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+        RCHK(dwhip_pass_c(CM_ZCONVJ, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
+        RCHK(dwhip_pass_c(CM_YINV, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+        RCHK(dwhip_pass_x(XM_WEIGHTS, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
+    }
+    return DWHIP_OK;
+}
+
+// One round (iteration k) of a single job; `full` = also run the adjoint
+// half (F, G, H), which only the next iteration's update consumes.
+int dwhip_job_round(const DwhipJob* j, bool full, hipStream_t s)
+{
+    const DwhipTiles& t = j->tiles;
+    RCHK(dwhip_pass_x(XM_UPDATE, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
+    RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+    RCHK(dwhip_pass_c(CM_ZCONV, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
+    RCHK(dwhip_pass_c(CM_YINV, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+    RCHK(dwhip_pass_x(XM_RATIO, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
+    if (full) {
+        RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+        RCHK(dwhip_pass_c(CM_ZCONVJ, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
+        RCHK(dwhip_pass_c(CM_YINV, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+    }
+    HIPCHK(dwhip_launch_round_end(j->bd, j->jd, 1, 1, s));
+    return DWHIP_OK;
+}
+
+// dw_iterator_next for REL / ABS (src/dw.c:235-268), given the error of the
+// last two iterations run.
+static bool iterator_continue(const dwhip_params* p, int iter, float error, float lasterror)
+{
+    if (p->iter_type == DWHIP_ITER_REL) {
+        if (iter < 2) lasterror = 2 * error * p->err_rel;
+        if (fabs(error - lasterror) / error < p->err_rel) return false;
+    } else if (p->iter_type == DWHIP_ITER_ABS) {
+        if (iter > 0 && error < p->err_abs) return false;
+    }
+    const int niter = p->iter_type == DWHIP_ITER_FIXED ? p->niter : p->maxiter;
+    return iter < niter;
+}
+
+static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                   int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, hipStream_t s)
+{
+    DwhipJob job;
+    const float* start = (prm->start != DWHIP_START_FLAT) ? prm->start_image : nullptr;
+    int rc = dwhip_job_create(&job, im, M, N, P, psf, pM, pN, pP, prm, start);
+    if (rc != DWHIP_OK) {
+        dwhip_job_free(&job);
+        return rc;
+    }
+    hipEvent_t e0, e1, e2;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventCreate(&e2);
+    hipEventRecord(e0, s);
+    rc = dwhip_job_setup(&job, s);
+    hipEventRecord(e1, s);
+
+    const bool fixed = prm->iter_type == DWHIP_ITER_FIXED;
+    int nrun = 0;
+    std::vector<float> errs(job.d.niter + 1, 0.f);
+    if (rc == DWHIP_OK) {
+        if (fixed) {
+            for (int k = 0; k < prm->niter && rc == DWHIP_OK; ++k) {
+                rc = dwhip_job_round(&job, k + 1 < prm->niter, s);
+                nrun = k + 1;
+            }
+        } else {
+            // the error of iteration k decides whether k+1 runs
+            float error = 1.f, lasterror = 1.f;
+            for (int k = 0; rc == DWHIP_OK; ++k) {
+                if (!iterator_continue(prm, k, error, lasterror)) break;
+                rc = dwhip_job_round(&job, true, s);
+                if (rc != DWHIP_OK) break;
+                nrun = k + 1;
+                float ek = 0;
+                hipError_t e = hipMemcpyAsync(&ek, job.d.err + k, sizeof(float), hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) rc = dwhip_hip_fail(e, "error readback", __LINE__);
+                lasterror = error;
+                error = ek;
+            }
+        }
+    }
+    hipEventRecord(e2, s);
+    if (rc == DWHIP_OK && nrun > 0) {
+        hipError_t e = dwhip_launch_crop(job.jd, 0, nrun - 1, out, 1024, s);
+        if (e != hipSuccess) rc = dwhip_hip_fail(e, "crop", __LINE__);
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess && rc == DWHIP_OK) rc = dwhip_hip_fail(e, "synchronize", __LINE__);
+    if (rc == DWHIP_OK && st) {
+        JobState hs;
+        hipMemcpy(&hs, job.d.st, sizeof(hs), hipMemcpyDeviceToHost);
+        st->bg = hs.bg;
+        st->iterations = nrun;
+        for (int d = 0; d < 3; ++d) {
+            st->work[d] = job.work[d];
+            st->ref_work[d] = job.ref_work[d];
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        st->setup_ms = ms;
+        hipEventElapsedTime(&ms, e1, e2);
+        st->iter_ms = ms;
+        if (st->err_trace && st->err_trace_len > 0 && nrun > 0) {
+            const int n = std::min(nrun, st->err_trace_len);
+            hipMemcpy(st->err_trace, job.d.err, n * sizeof(float), hipMemcpyDeviceToHost);
+        }
+        if (prm->verbosity > 0)
+            printf("dwhip: image %lldx%lldx%lld psf %lldx%lldx%lld job %lldx%lldx%lld (ref %lldx%lldx%lld), "
+                   "%d iterations, setup %.2f ms, iterations %.2f ms\n",
+                   (long long)M, (long long)N, (long long)P, (long long)pM, (long long)pN, (long long)pP,
+                   (long long)job.work[0], (long long)job.work[1], (long long)job.work[2],
+                   (long long)job.ref_work[0], (long long)job.ref_work[1], (long long)job.ref_work[2], nrun,
+                   st->setup_ms, st->iter_ms);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    dwhip_job_free(&job);
+    return rc;
+}
+
+static int check_args(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                      int64_t pP, const dwhip_params* prm, const float* out)
+{
+    if (!im || !psf || !prm || !out) return DWHIP_ERR_ARG;
+    if (M < 1 || N < 1 || P < 1 || pM < 1 || pN < 1 || pP < 1) return DWHIP_ERR_ARG;
+    if (prm->iter_type < DWHIP_ITER_ABS || prm->iter_type > DWHIP_ITER_FIXED) return DWHIP_ERR_ARG;
+    if (prm->metric != DWHIP_METRIC_MSE && prm->metric != DWHIP_METRIC_IDIV) return DWHIP_ERR_ARG;
+    if (prm->start < DWHIP_START_FLAT || prm->start > DWHIP_START_IDENTITY) return DWHIP_ERR_ARG;
+    if (prm->iter_type == DWHIP_ITER_FIXED && prm->niter < 0) return DWHIP_ERR_ARG;
+    if (prm->iter_type != DWHIP_ITER_FIXED && prm->maxiter < 0) return DWHIP_ERR_ARG;
+    return DWHIP_OK;
+}
+
+int dwhip_shb_run_device(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                         int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, void* stream)
+{
+    int rc = check_args(im, M, N, P, psf, pM, pN, pP, prm, out);
+    if (rc != DWHIP_OK) return rc;
+    rc = dwhip_init();
+    if (rc != DWHIP_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (prm->iter_type == DWHIP_ITER_FIXED && prm->niter == 0) {
+        // method_shb.c:101-105
+        HIPCHK(hipMemcpyAsync(out, im, (size_t)M * N * P * sizeof(float), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (st) {
+            st->bg = prm->bg;
+            st->iterations = 0;
+        }
+        return DWHIP_OK;
+    }
+    return run_job(im, M, N, P, psf, pM, pN, pP, prm, out, st, s);
+}
+
+int dwhip_shb_run(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                  int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st)
+{
+    int rc = check_args(im, M, N, P, psf, pM, pN, pP, prm, out);
+    if (rc != DWHIP_OK) return rc;
+    const size_t n = (size_t)M * N * P, np = (size_t)pM * pN * pP;
+    if (prm->iter_type == DWHIP_ITER_FIXED && prm->niter == 0) {
+        memcpy(out, im, n * sizeof(float));  // method_shb.c:101-105
+        if (st) {
+            st->bg = prm->bg;
+            st->iterations = 0;
+        }
+        return DWHIP_OK;
+    }
+    rc = dwhip_init();
+    if (rc != DWHIP_OK) return rc;
+    const bool has_start = prm->start != DWHIP_START_FLAT && prm->start_image;
+    float *dim = nullptr, *dpsf = nullptr, *dout = nullptr, *dstart = nullptr;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipMalloc(&dim, n * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dpsf, np * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dout, n * sizeof(float));
+    if (e == hipSuccess && has_start) e = hipMalloc(&dstart, n * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(dim, im, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dpsf, psf, np * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && has_start) e = hipMemcpy(dstart, prm->start_image, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rc = dwhip_hip_fail(e, "upload", __LINE__);
+    } else {
+        dwhip_params p2 = *prm;
+        p2.start_image = dstart;
+        rc = run_job(dim, M, N, P, dpsf, pM, pN, pP, &p2, dout, st, s);
+        if (rc == DWHIP_OK) {
+            e = hipMemcpy(out, dout, n * sizeof(float), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = dwhip_hip_fail(e, "download", __LINE__);
+        }
+    }
+    if (dim) hipFree(dim);
+    if (dpsf) hipFree(dpsf);
+    if (dout) hipFree(dout);
+    if (dstart) hipFree(dstart);
+    if (s) hipStreamDestroy(s);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// FFT check entry
+// ---------------------------------------------------------------------------
+int dwhip_fft3_device(float* vol, float* spec, int64_t nx, int64_t ny, int64_t nz, int inverse, void* stream)
+{
+    if (!vol || !spec) return DWHIP_ERR_ARG;
+    int64_t w[3] = {nx, ny, nz};
+    if (!size_ok(w)) return DWHIP_ERR_SIZE;
+    int rc = dwhip_init();
+    if (rc != DWHIP_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    // a job whose U0 is `vol`; C is its own buffer, copied to/from `spec`
+    dwhip_params prm;
+    dwhip_params_default(&prm, 1);
+    prm.border_quality = 0;
+    prm.work[0] = nx;
+    prm.work[1] = ny;
+    prm.work[2] = nz;
+    DwhipJob job;
+    rc = dwhip_job_create(&job, vol, nx, ny, nz, vol, 1, 1, 1, &prm, nullptr);
+    if (rc != DWHIP_OK) {
+        dwhip_job_free(&job);
+        return rc;
+    }
+    const JobDev& d = job.d;
+    // point U0 at the caller's volume
+    JobDev d2 = d;
+    d2.U0 = vol;
+    hipError_t e = hipMemcpy(job.jd, &d2, sizeof(d2), hipMemcpyHostToDevice);
+    const DwhipTiles& t = job.tiles;
+    const size_t rowb = (size_t)d.hc * sizeof(float2), pitch = (size_t)d.hcs * sizeof(float2);
+    if (e == hipSuccess) {
+        if (!inverse) {
+            rc = dwhip_pass_x(XM_LOAD, &job, job.bd, job.jd, t.ntile[KIND_X], t.lds[KIND_X], s);
+            if (!rc) rc = dwhip_pass_c(CM_YFWD, &job, job.bd, job.jd, t.ntile[KIND_Y], t.lds[KIND_Y], s);
+            if (!rc) rc = dwhip_pass_c(CM_ZFWD, &job, job.bd, job.jd, t.ntile[KIND_Z], t.lds[KIND_Z], s);
+            if (!rc) e = hipMemcpy2DAsync(spec, rowb, d.C, pitch, rowb, (size_t)ny * nz, hipMemcpyDeviceToDevice, s);
+        } else {
+            e = hipMemcpy2DAsync(d.C, pitch, spec, rowb, rowb, (size_t)ny * nz, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) {
+                rc = dwhip_pass_c(CM_ZINV, &job, job.bd, job.jd, t.ntile[KIND_Z], t.lds[KIND_Z], s);
+                if (!rc) rc = dwhip_pass_c(CM_YINV, &job, job.bd, job.jd, t.ntile[KIND_Y], t.lds[KIND_Y], s);
+                if (!rc) rc = dwhip_pass_x(XM_STORE, &job, job.bd, job.jd, t.ntile[KIND_X], t.lds[KIND_X], s);
+            }
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess && rc == DWHIP_OK) rc = dwhip_hip_fail(e, "fft3", __LINE__);
+    dwhip_job_free(&job);
+    return rc;
+}

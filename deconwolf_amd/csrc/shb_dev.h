@@ -1,0 +1,74 @@
+// Device-side job table of the SHB engine.
+//
+// One JobDev describes one deconvolution job (a whole image or one tile).
+// Every pass kernel receives the table and walks a shared tile space: the
+// tiles of job j of pass kind k are [tile0[k], tile0[k] + ntile[k]).  A plain
+// per-image run is a table with one job; the batched/served engine puts many
+// jobs of different shapes in one table, so one launch per pass serves all
+// of them.
+#pragma once
+#include <stdint.h>
+#include "fft_lds.h"
+
+enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2, KIND_N = 3 };
+
+// metric / start values mirror dw_metric and dw_start_condition (src/dw.h:64-77)
+enum { DWHIP_DEV_METRIC_MSE = 0, DWHIP_DEV_METRIC_IDIV = 1 };
+
+// State written on the device: by the setup statistics kernel (bg, c0,
+// mid) and by the end-of-round kernel (iter).  JobDev copies in several
+// tables share it through the pointer.
+struct JobState {
+    float bg;           // background level (method_shb.c:108-117)
+    float c0;           // flat start constant (method_shb.c:255-268)
+    int mid[3];         // PSF argmax (method_shb.c:195)
+    int iter;           // current iteration
+    int pad[2];
+};
+
+struct JobDev {
+    // transform (work) size; h = nx/2, hc = h+1 complex per x-row, hcs = row stride
+    int nx, ny, nz, h, hc, hcs;
+    // image and PSF sizes
+    int M, N, P;
+    int pM, pN, pP;
+    // reference work size volume, divides the flat start (method_shb.c:266)
+    double refMNP;
+    // tile bookkeeping per pass kind
+    int ntile[KIND_N];
+    int tile0[KIND_N];
+    int xrows;          // x-rows per X tile
+    int ycols, zcols;   // columns per Y / Z tile
+    // options
+    int positivity;
+    int metric;
+    int has_w;          // Bertero weights in use (bq > 0)
+    int bg_auto;
+    int start_identity; // start from start_img (or im) inserted, instead of flat
+    int niter;          // iterations requested
+    float alphamax;
+    float bg_user;
+    JobState* st;       // device-written state of this job
+    // buffers
+    float2* C;          // complex work volume, [nz][ny][hcs]
+    float2* K;          // OTF / (nx*ny*nz), [nz][ny][hcs]
+    float* U0;          // x_k lives in U[k & 1]
+    float* U1;
+    float* W;           // Bertero weights, nx*ny*nz (unused when has_w == 0)
+    const float* im;    // image M*N*P (offset already added)
+    const float* psf;   // PSF pM*pN*pP
+    const float* start_img;  // M*N*P start image for identity/LP starts
+    double* part;       // per X-tile partial sums (ntile[KIND_X])
+    double* stat;       // setup reduction scratch
+    float* err;         // per-iteration error, niter floats
+    double* errd;       // per-iteration error in double, niter
+    // FFT plans: complex lengths h (x rows), ny, nz; twiddles relative to tw
+    FftPlanDev px, py, pz;
+    const float2* tw;
+    const float2* rtw;  // exp(-2 pi i k / nx), k < h
+};
+
+struct BatchDev {
+    int njobs;
+    int ntile[KIND_N];
+};

@@ -1,0 +1,24 @@
+// Host-side launchers of the SHB pass kernels (shb_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "shb_dev.h"
+
+// X-pass modes: PSF (shifted PSF rows, R2C), ONES (image box, R2C),
+// WEIGHTS (C2R, Bertero weights), UPDATE (pass A), RATIO (pass E),
+// LOAD (R2C of U0), STORE (C2R into U0, unnormalised).
+enum { XM_PSF = 0, XM_ONES = 1, XM_WEIGHTS = 2, XM_UPDATE = 3, XM_RATIO = 4, XM_LOAD = 5, XM_STORE = 6 };
+// Column-pass modes; ZOTF: forward, divide by nx*ny*nz, store into K.
+enum { CM_YFWD = 0, CM_YINV = 1, CM_ZOTF = 2, CM_ZCONV = 3, CM_ZCONVJ = 4, CM_ZFWD = 5, CM_ZINV = 6 };
+
+#define DWHIP_NTHR 256
+#define DWHIP_XMAXE 16   // elements per thread per stage, X pass
+#define DWHIP_CMAXE 40   // elements per thread per stage, Y/Z passes
+
+extern "C" {
+hipError_t dwhip_launch_xpass(int mode, const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s);
+hipError_t dwhip_launch_colpass(int mode, const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s);
+hipError_t dwhip_launch_stats(const JobDev* jobs, int j, int nblk, hipStream_t s);
+hipError_t dwhip_launch_start(const JobDev* jobs, int j, int grid, hipStream_t s);
+hipError_t dwhip_launch_round_end(const BatchDev* bd, const JobDev* jobs, int njobs, int advance, hipStream_t s);
+hipError_t dwhip_launch_crop(const JobDev* jobs, int j, int k, float* out, int grid, hipStream_t s);
+}

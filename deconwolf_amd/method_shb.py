@@ -1,0 +1,257 @@
+"""Python host side of the HIP SHB method.
+
+Mirrors deconwolf's method interface for the scaled-heavy-ball loop:
+
+* :class:`DwOpts` holds the `dw_opts` fields that `deconvolve_shb` reads
+  (src/dw.h:104-211) with the defaults of `dw_opts_new` (src/dw.c:340-413);
+* :func:`deconvolve_shb_hip` has the meaning of the `dw_function`
+  `deconvolve_shb(im, M, N, P, psf, pM, pN, pP, s)` (src/dw.h:89-91,
+  src/method_shb.c:77-424): same inputs (image with the offset already
+  added, PSF normalised and autocropped), same output (the x_{n-1} iterate
+  cropped to M x N x P), and it writes the resolved background back into
+  ``s.bg`` as the reference does (method_shb.c:108-117);
+* :func:`deconvolve` mirrors the in-memory `dw_run` path used by the Python
+  wrapper (src/pywrapper/src/pydeconwolf.c:284-395, src/dw.c:2799-2852):
+  normalise, autocrop, add the offset, run the method.
+
+Volumes are numpy arrays shaped (P, N, M), x fastest, like pydeconwolf.
+All arithmetic runs in libdwhip on the GPU; this module only marshals.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import (DwhipParams, DwhipStats, check, lib, ITER_ABS, ITER_REL, ITER_FIXED,
+                   METRIC_MSE, METRIC_IDIV, START_FLAT, START_LP, START_IDENTITY)
+
+
+@dataclass
+class DwOpts:
+    """The dw_opts fields on the SHB path (src/dw.h:104-211)."""
+
+    nIter: int = 1                    # dw.c:349
+    iter_type: int = ITER_REL         # dw.c:361
+    maxiter: int = 250                # dw.c:350
+    err_rel: float = 0.02             # dw.c:351
+    err_abs: float = 1.0              # dw.c:352
+    bg: float = 1e-2                  # dw.c:389
+    bg_auto: int = 1                  # dw.c:390
+    borderQuality: int = 2            # dw.c:382
+    positivity: int = 1               # dw.c:388
+    alphamax: float = 1.0             # dw.c:401
+    metric: int = METRIC_IDIV         # dw.c:398
+    start_condition: int = START_FLAT  # dw.c:410
+    offset: float = 5.0               # dw.c:394
+    xycropfactor: float = 0.001       # dw.c:377
+    verbosity: int = 0
+    # transform size; None = automatic (smallest 2,3,5,7-smooth size >= the
+    # reference work size for borderQuality 2)
+    work: Optional[Tuple[int, int, int]] = None
+    # filled by the method: error per iteration, stats
+    err_trace: Optional[np.ndarray] = field(default=None, repr=False)
+    stats: Optional[dict] = field(default=None, repr=False)
+
+
+def _params(s: DwOpts) -> DwhipParams:
+    p = DwhipParams()
+    lib().dwhip_params_default(ctypes.byref(p), int(s.nIter))
+    p.niter = int(s.nIter)
+    p.iter_type = int(s.iter_type)
+    p.maxiter = int(s.maxiter)
+    p.err_rel = float(s.err_rel)
+    p.err_abs = float(s.err_abs)
+    p.bg = float(s.bg)
+    p.bg_auto = int(s.bg_auto)
+    p.border_quality = int(s.borderQuality)
+    p.positivity = int(s.positivity)
+    p.alphamax = float(s.alphamax)
+    p.metric = int(s.metric)
+    p.start = int(s.start_condition)
+    p.start_image = None
+    w = s.work or (0, 0, 0)
+    for i in range(3):
+        p.work[i] = int(w[i])
+    p.verbosity = int(s.verbosity)
+    return p
+
+
+def _stats_dict(st: DwhipStats) -> dict:
+    return {
+        "bg": st.bg, "iterations": st.iterations,
+        "work": tuple(st.work), "ref_work": tuple(st.ref_work),
+        "setup_ms": st.setup_ms, "iter_ms": st.iter_ms,
+    }
+
+
+def work_size(imshape, psfshape, borderQuality=2):
+    """(work, ref_work) as (wM, wN, wP) tuples for (P,N,M) / (pP,pN,pM) shapes."""
+    P, N, M = imshape
+    pP, pN, pM = psfshape
+    w = (ctypes.c_int64 * 3)()
+    r = (ctypes.c_int64 * 3)()
+    rc = lib().dwhip_work_size(M, N, P, pM, pN, pP, borderQuality, w, r)
+    return tuple(w), tuple(r), rc
+
+
+def _finish(s, st, err):
+    n = st.iterations
+    s.err_trace = err[:n].copy()
+    s.stats = _stats_dict(st)
+    if s.bg_auto and n > 0:
+        s.bg = float(st.bg)
+
+
+def _lp_start(im, s):
+    if s.start_condition == START_LP:
+        from .filters import gsmooth
+        return gsmooth(np.array(im, dtype=np.float32, copy=True), 8.0)
+    return None
+
+
+def deconvolve_shb_hip(im: np.ndarray, psf: np.ndarray, s: DwOpts) -> np.ndarray:
+    """SHB deconvolution on the GPU, host arrays in and out.
+
+    im: (P, N, M) float32 image, offset already added (borrowed).
+    psf: (pP, pN, pM) float32 PSF, sum 1, autocropped.
+    Returns a new (P, N, M) float32 array: x_{n-1} after n iterations
+    (method_shb.c:385-413)."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    psf = np.ascontiguousarray(psf, dtype=np.float32)
+    if im.ndim != 3 or psf.ndim != 3:
+        raise ValueError("im and psf must be 3-D (P, N, M) arrays")
+    P, N, M = im.shape
+    pP, pN, pM = psf.shape
+    p = _params(s)
+    start = _lp_start(im, s)
+    if start is not None:
+        p.start_image = start.ctypes.data
+    out = np.empty_like(im)
+    nerr = max(int(s.nIter), int(s.maxiter), 1)
+    err = np.zeros(nerr, dtype=np.float32)
+    st = DwhipStats()
+    st.err_trace = err.ctypes.data
+    st.err_trace_len = nerr
+    check(lib().dwhip_shb_run(im.ctypes.data, M, N, P, psf.ctypes.data, pM, pN, pP,
+                              ctypes.byref(p), out.ctypes.data, ctypes.byref(st)),
+          "dwhip_shb_run")
+    _finish(s, st, err)
+    return out
+
+
+def deconvolve_shb_hip_device(im, psf, s: DwOpts, out=None, start_image=None):
+    """Same as :func:`deconvolve_shb_hip` for torch tensors already on the
+    GPU (float32, contiguous, shapes (P,N,M) / (pP,pN,pM)).  Work is queued
+    on torch's current stream.  Returns `out` (allocated if None)."""
+    import torch
+    if not (im.is_cuda and psf.is_cuda):
+        raise ValueError("im and psf must be GPU tensors")
+    im = im.contiguous()
+    psf = psf.contiguous()
+    if im.dtype != torch.float32 or psf.dtype != torch.float32:
+        raise ValueError("im and psf must be float32")
+    P, N, M = im.shape
+    pP, pN, pM = psf.shape
+    if out is None:
+        out = torch.empty_like(im)
+    p = _params(s)
+    if start_image is not None:
+        start_image = start_image.contiguous()
+        p.start_image = start_image.data_ptr()
+    elif s.start_condition == START_LP:
+        raise ValueError("LP start on the device path needs start_image (the smoothed image)")
+    nerr = max(int(s.nIter), int(s.maxiter), 1)
+    err = np.zeros(nerr, dtype=np.float32)
+    st = DwhipStats()
+    st.err_trace = err.ctypes.data
+    st.err_trace_len = nerr
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().dwhip_shb_run_device(im.data_ptr(), M, N, P, psf.data_ptr(), pM, pN, pP,
+                                     ctypes.byref(p), out.data_ptr(), ctypes.byref(st), stream),
+          "dwhip_shb_run_device")
+    _finish(s, st, err)
+    return out
+
+
+def fft3_device(vol, inverse=False, spec=None):
+    """Unnormalised 3-D R2C (or C2R with inverse=True) through the engine's
+    transforms.  vol: float32 (nz, ny, nx) GPU tensor, nx even; spec:
+    complex64 (nz, ny, nx//2+1) GPU tensor.  Returns spec (R2C) or vol (C2R)."""
+    import torch
+    nz, ny, nx = vol.shape
+    if spec is None:
+        spec = torch.empty((nz, ny, nx // 2 + 1), dtype=torch.complex64, device=vol.device)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().dwhip_fft3_device(vol.data_ptr(), spec.data_ptr(), nx, ny, nz, 1 if inverse else 0, stream),
+          "dwhip_fft3_device")
+    return vol if inverse else spec
+
+
+# ---------------------------------------------------------------------------
+# caller-side steps of dw_run (src/dw.c:2799-2852) for the in-memory API
+# ---------------------------------------------------------------------------
+def fim_normalize_sum1(psf):
+    """psf / sum(psf), sum in double (src/fim.c:900-912)."""
+    psf = np.asarray(psf, dtype=np.float32)
+    s = float(np.sum(psf, dtype=np.float64))
+    return (psf.astype(np.float64) / s).astype(np.float32)
+
+
+def psf_autocrop(psf, imshape, s: DwOpts):
+    """psf_autocrop (src/dw.c:1973-1997): crop to at most 2*dim-1 per axis
+    (psf_autocrop_byImage, dw.c:1764-1855), then drop outer x/y planes
+    whose plane sum is below xycropfactor * the largest plane sum
+    (psf_autocrop_XY, dw.c:1857-1971).  Plane sums are taken in float64."""
+    P, N, M = imshape
+    psf = np.asarray(psf, dtype=np.float32)
+    p, n, m = psf.shape
+    mopt, nopt, popt = (M - 1) * 2 + 1, (N - 1) * 2 + 1, (P - 1) * 2 + 1
+    if s.borderQuality == 0:
+        mopt, nopt, popt = M, N, P
+    if not p < popt and (m > mopt or n > nopt or p > popt):
+        m0, m1, n0, n1, p0, p1 = 0, m - 1, 0, n - 1, 0, p - 1
+        if m > mopt:
+            m0, m1 = (m - mopt) // 2, m1 - (m - mopt) // 2
+        if n > nopt:
+            n0, n1 = (n - nopt) // 2, n1 - (n - nopt) // 2
+        if p > popt:
+            p0, p1 = (p - popt) // 2, p1 - (p - popt) // 2
+        psf = np.ascontiguousarray(psf[p0:p1 + 1, n0:n1 + 1, m0:m1 + 1])
+    if s.borderQuality > 0 and s.xycropfactor > 0:
+        sums = psf.astype(np.float64).sum(axis=(0, 1))  # one sum per x
+        maxsum = max(0.0, float(sums.max()))
+        first = -1
+        ssum = 0.0
+        while ssum < s.xycropfactor * maxsum:
+            first += 1
+            ssum = float(sums[first])
+        if first >= 1:
+            pM = psf.shape[2]
+            pN = psf.shape[1]
+            psf = np.ascontiguousarray(psf[:, first:pN - first, first:pM - first])
+    return psf
+
+
+def deconvolve(image, psf, iterations, bq=2, verbose=0, opts: Optional[DwOpts] = None):
+    """In-memory deconvolution like pydeconwolf.deconvolve
+    (src/pywrapper/src/pydeconwolf.c:284-395): FIXED iteration count, PSF
+    normalised and autocropped, offset added before the method; the result
+    keeps the offset, as dw_run's non-tiled path returns it (dw.c:2845-2849)."""
+    s = opts or DwOpts()
+    s.nIter = int(iterations)
+    s.iter_type = ITER_FIXED
+    s.borderQuality = int(bq)
+    s.verbosity = int(verbose)
+    im = np.array(image, dtype=np.float32, copy=True)
+    if im.min() < 0:
+        raise ValueError("the image has negative values (dw.c:2703-2710)")
+    if im.max() < 1:
+        raise ValueError("the image maximum is below 1 (dw.c:2711-2721)")
+    ps = fim_normalize_sum1(psf)
+    ps = psf_autocrop(ps, im.shape, s)
+    ps = fim_normalize_sum1(ps)
+    if s.offset > 0:
+        im += np.float32(s.offset)
+    return deconvolve_shb_hip(im, ps, s)

@@ -1,0 +1,149 @@
+/*
+ * dwhip: C-ABI of the MI355X (gfx950) scaled-heavy-ball Richardson-Lucy
+ * (SHB) deconvolution engine.
+ *
+ * The engine is a drop-in for deconwolf's CPU method `deconvolve_shb`
+ * (src/method_shb.c:77-424), whose signature is the plugin type
+ * `dw_function` (src/dw.h:89-91).  A deconwolf method module
+ * (deconwolf_amd/method_shb_hip.c) maps `dw_opts` onto `dwhip_params` and
+ * calls dwhip_shb_run().  Plain pointers and sizes only: no HIP, torch or
+ * deconwolf types appear here.
+ *
+ * Arrays are x-fastest: element (m, n, p) of an M x N x P volume sits at
+ * m + n*M + p*M*N (src/dw.c:79-85).
+ */
+#ifndef DWHIP_H
+#define DWHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define DWHIP_OK 0
+#define DWHIP_ERR_ARG (-1)       /* invalid argument */
+#define DWHIP_ERR_SIZE (-2)      /* transform size not supported */
+#define DWHIP_ERR_HIP (-3)       /* HIP runtime error */
+#define DWHIP_ERR_NOMEM (-4)     /* device allocation failed / over budget */
+#define DWHIP_ERR_PSF (-5)       /* PSF maximum not movable to the origin (method_shb.c:199-203) */
+#define DWHIP_ERR_NODEVICE (-6)  /* no usable gfx950 device */
+#define DWHIP_ERR_STATE (-7)     /* call not valid in the object's state */
+
+/* ---- option values, numerically equal to deconwolf's enums ---------- */
+/* dw_iter_type (src/dw.h:79-83) */
+#define DWHIP_ITER_ABS 0
+#define DWHIP_ITER_REL 1
+#define DWHIP_ITER_FIXED 2
+/* dw_metric (src/dw.h:74-77) */
+#define DWHIP_METRIC_MSE 0
+#define DWHIP_METRIC_IDIV 1
+/* dw_start_condition (src/dw.h:64-72) */
+#define DWHIP_START_FLAT 0
+#define DWHIP_START_LP 1
+#define DWHIP_START_IDENTITY 2
+
+/*
+ * The dw_opts fields deconvolve_shb reads (src/dw.h:104-211), resolved to
+ * plain values by the caller.
+ */
+typedef struct {
+    int niter;           /* nIter: iterations for DWHIP_ITER_FIXED */
+    int iter_type;       /* iter_type */
+    int maxiter;         /* maxiter: cap for REL / ABS */
+    float err_rel;       /* err_rel */
+    float err_abs;       /* err_abs */
+    float bg;            /* bg, used when bg_auto == 0 */
+    int bg_auto;         /* bg_auto: bg = max(min(im), 1e-2) (method_shb.c:108-117) */
+    int border_quality;  /* borderQuality: 0, 1 or 2 (method_shb.c:137-153) */
+    int positivity;      /* positivity: clamp x to bg after each update */
+    float alphamax;      /* alphamax: cap of the heavy-ball weight */
+    int metric;          /* DWHIP_METRIC_* */
+    int start;           /* DWHIP_START_*; LP passes the smoothed image as start_image */
+    const float *start_image; /* M*N*P start for IDENTITY/LP, NULL = im (same memory space as im) */
+    int64_t work[3];     /* transform size; zeros = automatic (see dwhip_work_size) */
+    int verbosity;       /* 0 silent; > 0 prints sizes and timings to stdout */
+} dwhip_params;
+
+/* Results besides the output volume. */
+typedef struct {
+    float bg;              /* background level used (written back to s->bg by the caller) */
+    int iterations;        /* iterations run */
+    int64_t work[3];       /* transform size used */
+    int64_t ref_work[3];   /* deconwolf's work size for these inputs */
+    double setup_ms;       /* OTF, weights and start guess */
+    double iter_ms;        /* all iterations */
+    float *err_trace;      /* caller buffer (host), one error per iteration, or NULL */
+    int err_trace_len;     /* its length */
+} dwhip_stats;
+
+/* Fills p with deconwolf's defaults (dw_opts_new, src/dw.c:340-413) and
+ * niter iterations in FIXED mode. */
+void dwhip_params_default(dwhip_params *p, int niter);
+
+/* Text for a status code. */
+const char *dwhip_strerror(int code);
+
+/* Selects device 0 and checks it is usable.  Called lazily by the entry
+ * points; returns DWHIP_OK or DWHIP_ERR_NODEVICE. */
+int dwhip_init(void);
+
+/*
+ * Transform size for an M*N*P image and a pM*pN*pP PSF.  ref_work receives
+ * deconwolf's work size (method_shb.c:137-153); work receives the size the
+ * engine transforms: for border_quality 2 the smallest size >= ref_work
+ * with an even x extent whose halves and the y, z extents factor into 2, 3,
+ * 5 and 7 (the image region is unchanged by that padding); for quality 0
+ * and 1 the reference size itself, which must already factor that way.
+ * Returns DWHIP_OK or DWHIP_ERR_SIZE.
+ */
+int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int64_t pP,
+                    int border_quality, int64_t *work, int64_t *ref_work);
+
+/* Device bytes one job of these sizes needs. */
+size_t dwhip_job_bytes(const int64_t *work, int64_t M, int64_t N, int64_t P, int64_t pM,
+                       int64_t pN, int64_t pP, int border_quality, int niter);
+
+/*
+ * SHB deconvolution, host buffers: the drop-in for deconvolve_shb
+ * (src/method_shb.c:77-424, called through dw_function at src/dw.c:2841 and
+ * src/dw.c:2137).
+ *   im   M*N*P image with deconwolf's offset already added (borrowed)
+ *   psf  pM*pN*pP PSF, normalised to sum 1 and autocropped (borrowed; the
+ *        adapter frees it, as deconvolve_shb does at method_shb.c:191)
+ *   out  M*N*P output; receives the iterate deconvolve_shb returns: with
+ *        n iterations run, x_{n-1} (swap-back at method_shb.c:385-390);
+ *        with niter == 0 in FIXED mode a copy of im (method_shb.c:101-105)
+ * Returns a DWHIP_* status.
+ */
+int dwhip_shb_run(const float *im, int64_t M, int64_t N, int64_t P, const float *psf, int64_t pM,
+                  int64_t pN, int64_t pP, const dwhip_params *params, float *out,
+                  dwhip_stats *stats);
+
+/*
+ * Same, with im, psf, start_image and out in device memory, work queued on
+ * `stream` (a hipStream_t, NULL = default stream); returns after the
+ * result is in out.  No counterpart in deconwolf: it is the entry for
+ * callers whose volumes are already resident on the GPU.
+ */
+int dwhip_shb_run_device(const float *im, int64_t M, int64_t N, int64_t P, const float *psf,
+                         int64_t pM, int64_t pN, int64_t pP, const dwhip_params *params,
+                         float *out, dwhip_stats *stats, void *stream);
+
+/*
+ * FFT check entry (device memory): unnormalised 3-D R2C of an nx*ny*nz
+ * real volume (nx even) into (nx/2+1)*ny*nz complex, interleaved, the
+ * layout of fftwf_plan_dft_r2c_3d(nz, ny, nx) that src/fft.c:236-290
+ * plans; inverse != 0 runs the C2R direction instead (spec -> vol).
+ * No counterpart in deconwolf's method API; it exposes the transforms the
+ * loop uses so that they can be checked alone.
+ */
+int dwhip_fft3_device(float *vol, float *spec, int64_t nx, int64_t ny, int64_t nz, int inverse,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
