@@ -1,7 +1,12 @@
 """Builds libdwhip.so in-tree with hipcc for gfx950.
 
-    python -m deconwolf_amd.build
+    python -m deconwolf_amd.build [-v] [-j N]
+
+Every pass-kernel mode is its own translation unit (k_xpass.hip and
+k_colpass.hip compiled once per mode), compiled in parallel, then linked.
+Objects go to deconwolf_amd/_build/ (not tracked).
 """
+import concurrent.futures as cf
 import os
 import subprocess
 import sys
@@ -9,42 +14,77 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdwhip.so")
 
-SOURCES = ["shb_kernels.hip", "batch.cpp", "dwhip.cpp"]
-HEADERS = ["fft_lds.h", "shb_dev.h", "shb_kernels.h", "dwhip_internal.h"]
+N_XMODES = 7
+N_CMODES = 7
 
 # -packed-fp32-ops: no packed FP32 instructions in device code.  The host
 # pass of hipcc warns that it does not know the feature; that is harmless.
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
 
 def hipcc():
-    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
-        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
             return cand
     return "hipcc"
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    cmd = [hipcc()] + FLAGS + ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC] + srcs + ["-o", LIB + ".tmp"]
-    if verbose:
-        print(" ".join(cmd))
+def units():
+    """(source, extra defines, object name) for every translation unit."""
+    u = []
+    for m in range(N_XMODES):
+        u.append(("k_xpass.hip", ["-DXMODE=%d" % m], "k_xpass_%d.o" % m))
+    for m in range(N_CMODES):
+        u.append(("k_colpass.hip", ["-DCMODE=%d" % m], "k_colpass_%d.o" % m))
+    u.append(("k_misc.hip", [], "k_misc.o"))
+    for src in ("dwhip.cpp", "batch.cpp"):
+        if os.path.exists(os.path.join(CSRC, src)):
+            u.append((src, [], src.replace(".cpp", ".o")))
+    return u
+
+
+def _filter(stderr):
+    return "\n".join(l for l in stderr.splitlines() if "packed-fp32-ops' is not a recognized" not in l)
+
+
+def _compile(unit, verbose):
+    src, defs, obj = unit
+    cmd = [hipcc()] + FLAGS + defs + ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-c",
+                                     os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
-    errs = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops' is not a recognized" not in l)
+    msg = _filter(r.stderr)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + errs)
-    if verbose and errs.strip():
-        print(errs)
+        raise RuntimeError("hipcc failed on %s %s:\n%s" % (src, " ".join(defs), msg))
+    if verbose and msg.strip():
+        print(msg)
+    return obj
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    us = units()
+    jobs = jobs or min(len(us), os.cpu_count() or 4, 16)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda u: _compile(u, verbose), us))
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + \
+          [os.path.join(OBJ, o) for o in objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + _filter(r.stderr))
     os.replace(LIB + ".tmp", LIB)
     return LIB
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
+    j = None
+    if "-j" in sys.argv:
+        j = int(sys.argv[sys.argv.index("-j") + 1])
+    build(verbose="-v" in sys.argv, jobs=j)
     print(LIB)

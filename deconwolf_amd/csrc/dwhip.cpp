@@ -85,17 +85,20 @@ static bool is_smooth(int64_t n)
     return n == 1;
 }
 
+std::vector<int> dwhip_radices(int n, bool rows);
+
+// smallest length >= n with a column plan
 static int64_t next_smooth(int64_t n)
 {
-    while (!is_smooth(n)) ++n;
+    while (!(is_smooth(n) && (n == 1 || !dwhip_radices((int)n, false).empty()))) ++n;
     return n;
 }
 
-// even nx with nx/2 smooth
+// smallest even length >= n whose half has a row plan
 static int64_t next_smooth_even(int64_t n)
 {
     int64_t h = (n + 1) / 2;
-    while (!is_smooth(h)) ++h;
+    while (!(is_smooth(h) && (h == 1 || !dwhip_radices((int)h, true).empty()))) ++h;
     return 2 * h;
 }
 
@@ -123,9 +126,12 @@ static bool size_ok(const int64_t* w)
     if (w[0] % 2 != 0 || !is_smooth(w[0] / 2) || !is_smooth(w[1]) || !is_smooth(w[2])) return false;
     // kernel limits: at most DWHIP_XMAXE*NTHR complex per x-row tile,
     // DWHIP_CMAXE*NTHR per column tile, 2^20 for the index arithmetic
-    if (w[0] / 2 > DWHIP_XMAXE * DWHIP_NTHR) return false;
-    if (w[1] > DWHIP_CMAXE * DWHIP_NTHR || w[2] > DWHIP_CMAXE * DWHIP_NTHR) return false;
+    if (w[0] / 2 > DWHIP_XMAXE * DWHIP_XTHR) return false;
+    if (w[1] > DWHIP_CMAXE * DWHIP_CTHR || w[2] > DWHIP_CMAXE * DWHIP_CTHR) return false;
     if (w[0] < 2 || w[1] < 1 || w[2] < 1) return false;
+    if (w[0] / 2 > 1 && dwhip_radices((int)(w[0] / 2), true).empty()) return false;
+    if (w[1] > 1 && dwhip_radices((int)w[1], false).empty()) return false;
+    if (w[2] > 1 && dwhip_radices((int)w[2], false).empty()) return false;
     return true;
 }
 
@@ -151,26 +157,51 @@ int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int
     return size_ok(w) ? DWHIP_OK : DWHIP_ERR_SIZE;
 }
 
-// Radices of a 7-smooth n: 8s, then 4 or 2, then 5s, 3s, 7s.
-static std::vector<int> radices(int n)
+// Radices for a 2,3,5,7-smooth n: fewest stages, then the smallest largest
+// radix (fewer registers per butterfly), from the radices the kernels
+// instantiate (DWHIP_FOR_RADICES in fft_engine.h).
+static const int kRadicesCols[] = {16, 15, 9, 8, 7, 6, 5, 4, 3, 2};
+static const int kRadicesRows[] = {25, 20, 16, 15, 9, 8, 7, 6, 5, 4, 3, 2};
+
+static void radix_search(int n, bool rows, std::vector<int>& cur, std::vector<int>& best)
 {
-    std::vector<int> r;
-    int m = n;
-    while (m % 8 == 0) { r.push_back(8); m /= 8; }
-    if (m % 4 == 0) { r.push_back(4); m /= 4; }
-    if (m % 2 == 0) { r.push_back(2); m /= 2; }
-    while (m % 5 == 0) { r.push_back(5); m /= 5; }
-    while (m % 3 == 0) { r.push_back(3); m /= 3; }
-    while (m % 7 == 0) { r.push_back(7); m /= 7; }
-    return r;
+    if (n == 1) {
+        if (best.empty() || cur.size() < best.size() ||
+            (cur.size() == best.size() && *std::max_element(cur.begin(), cur.end()) <
+                                              *std::max_element(best.begin(), best.end())))
+            best = cur;
+        return;
+    }
+    if (cur.size() >= DWHIP_MAX_FFT_STAGES) return;
+    if (!best.empty() && cur.size() + 1 > best.size()) return;
+    const int* rs = rows ? kRadicesRows : kRadicesCols;
+    const int nr = rows ? (int)(sizeof(kRadicesRows) / sizeof(int)) : (int)(sizeof(kRadicesCols) / sizeof(int));
+    for (int i = 0; i < nr; ++i) {
+        const int r = rs[i];
+        // non-increasing radices: each factorisation is visited once
+        if (!cur.empty() && r > cur.back()) continue;
+        if (n % r != 0) continue;
+        cur.push_back(r);
+        radix_search(n / r, rows, cur, best);
+        cur.pop_back();
+    }
 }
 
-// Appends the plan's twiddles (forward sign) to tw, offsets relative to tw's start.
-int dwhip_make_plan(int n, FftPlanDev* p, std::vector<float2>& tw)
+// Radices for a 2,3,5,7-smooth n (rows: x-row transforms), at most
+// DWHIP_MAX_FFT_STAGES of them; empty when n has no such plan.
+std::vector<int> dwhip_radices(int n, bool rows)
+{
+    std::vector<int> cur, best;
+    if (n > 1) radix_search(n, rows, cur, best);
+    return best;
+}
+
+// Builds a plan with the given radix order and appends its twiddles
+// (forward sign) to tw, offsets relative to tw's start.
+int dwhip_make_plan(int n, const std::vector<int>& r, FftPlanDev* p, std::vector<float2>& tw)
 {
     memset(p, 0, sizeof(*p));
     p->n = n;
-    std::vector<int> r = radices(n);
     if ((int)r.size() > DWHIP_MAX_STAGES) return DWHIP_ERR_SIZE;
     p->nst = (int)r.size();
     int ns = 1;
@@ -187,7 +218,18 @@ int dwhip_make_plan(int n, FftPlanDev* p, std::vector<float2>& tw)
             }
         ns *= R;
     }
-    return DWHIP_OK;
+    return ns == n || n == 1 ? DWHIP_OK : DWHIP_ERR_SIZE;
+}
+
+// forward (radices R1..Rk) and inverse (Rk..R1) plans of length n
+static int make_plan_pair(int n, bool rows, FftPlanDev* f, FftPlanDev* i, std::vector<float2>& tw)
+{
+    std::vector<int> r = dwhip_radices(n, rows);
+    if (n > 1 && r.empty()) return DWHIP_ERR_SIZE;
+    std::vector<int> rr(r.rbegin(), r.rend());
+    int rc = dwhip_make_plan(n, r, f, tw);
+    if (rc == DWHIP_OK) rc = dwhip_make_plan(n, rr, i, tw);
+    return rc;
 }
 
 static int pow2_floor(int v)
@@ -202,9 +244,9 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
 {
     const int h = (int)(w[0] / 2), ny = (int)w[1], nz = (int)w[2];
     const int hc = h + 1;
-    t->xrows = std::max(1, std::min(8, DWHIP_XMAXE * DWHIP_NTHR / h));
-    t->ycols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_NTHR / ny)));
-    t->zcols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_NTHR / nz)));
+    t->xrows = std::max(1, std::min(64, DWHIP_XMAXE * DWHIP_XTHR / h));
+    t->ycols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_CTHR / ny)));
+    t->zcols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_CTHR / nz)));
     t->ntile[KIND_X] = (ny * nz + t->xrows - 1) / t->xrows;
     t->ntile[KIND_Y] = ((hc + t->ycols - 1) / t->ycols) * nz;
     t->ntile[KIND_Z] = ((hc + t->zcols - 1) / t->zcols) * ny;
@@ -305,7 +347,8 @@ int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int
 
     // twiddles
     std::vector<float2> tw;
-    if (dwhip_make_plan(d.h, &d.px, tw) || dwhip_make_plan(d.ny, &d.py, tw) || dwhip_make_plan(d.nz, &d.pz, tw))
+    if (make_plan_pair(d.h, true, &d.pxf, &d.pxi, tw) || make_plan_pair(d.ny, false, &d.pyf, &d.pyi, tw) ||
+        make_plan_pair(d.nz, false, &d.pzf, &d.pzi, tw))
         return DWHIP_ERR_SIZE;
     const size_t rtw_off = tw.size();
     for (int k = 0; k < d.h; ++k) {

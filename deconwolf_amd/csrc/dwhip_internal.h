@@ -32,7 +32,8 @@ struct DwhipJob {
 };
 
 int dwhip_hip_fail(hipError_t e, const char* what, int line);
-int dwhip_make_plan(int n, FftPlanDev* p, std::vector<float2>& tw);
+int dwhip_make_plan(int n, const std::vector<int>& radices, FftPlanDev* p, std::vector<float2>& tw);
+std::vector<int> dwhip_radices(int n, bool rows);
 void dwhip_tile_config(const int64_t* w, DwhipTiles* t);
 int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P, const float* psf_dev,
                      int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm, const float* start_dev);

@@ -1,0 +1,237 @@
+// Set-up, reduction and crop kernels of the SHB loop, and the dispatch of
+// the per-mode pass launchers.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "shb_dev.h"
+#include "shb_kernels.h"
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NT / 64; ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Setup statistics: min(im) and sum(im) (bg_auto, flat start) and argmax(psf)
+// ---------------------------------------------------------------------------
+#define STHR 256
+// stat layout per job: [nblk][4] doubles: min, sum, max, argmax index
+__global__ __launch_bounds__(STHR) void k_stats_partial(const JobDev* __restrict__ jobs, int j, int nblk)
+{
+    const JobDev& J = jobs[j];
+    __shared__ double smin[STHR], ssum[STHR], smax[STHR], sidx[STHR];
+    const size_t n = (size_t)J.M * J.N * J.P;
+    const size_t np = (size_t)J.pM * J.pN * J.pP;
+    double vmin = INFINITY, vsum = 0, vmax = -INFINITY, vidx = 1e300;
+    for (size_t i = (size_t)blockIdx.x * STHR + threadIdx.x; i < n; i += (size_t)nblk * STHR) {
+        const double v = J.im[i];
+        vmin = v < vmin ? v : vmin;
+        vsum += v;
+    }
+    for (size_t i = (size_t)blockIdx.x * STHR + threadIdx.x; i < np; i += (size_t)nblk * STHR) {
+        const double v = J.psf[i];
+        if (v > vmax) {  // strictly greater keeps the lowest index per thread
+            vmax = v;
+            vidx = (double)i;
+        }
+    }
+    const int tid = threadIdx.x;
+    smin[tid] = vmin; ssum[tid] = vsum; smax[tid] = vmax; sidx[tid] = vidx;
+    __syncthreads();
+    for (int s = STHR / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            smin[tid] = fmin(smin[tid], smin[tid + s]);
+            ssum[tid] += ssum[tid + s];
+            const double m2 = smax[tid + s], i2 = sidx[tid + s];
+            if (m2 > smax[tid] || (m2 == smax[tid] && i2 < sidx[tid])) {
+                smax[tid] = m2;
+                sidx[tid] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double* o = J.stat + 4 * (size_t)blockIdx.x;
+        o[0] = smin[0]; o[1] = ssum[0]; o[2] = smax[0]; o[3] = sidx[0];
+    }
+}
+
+__global__ __launch_bounds__(STHR) void k_stats_final(const JobDev* __restrict__ jobs, int j, int nblk)
+{
+    const JobDev& J = jobs[j];
+    if (threadIdx.x != 0) return;
+    double vmin = INFINITY, vsum = 0, vmax = -INFINITY, vidx = 1e300;
+    for (int b = 0; b < nblk; ++b) {
+        const double* o = J.stat + 4 * (size_t)b;
+        vmin = fmin(vmin, o[0]);
+        vsum += o[1];
+        if (o[2] > vmax || (o[2] == vmax && o[3] < vidx)) {
+            vmax = o[2];
+            vidx = o[3];
+        }
+    }
+    JobState* st = J.st;
+    // bg_auto: bg = max(min(im), 1e-2)  (method_shb.c:108-117)
+    float bg = J.bg_user;
+    if (J.bg_auto) {
+        bg = (float)vmin;
+        if (bg < 1e-2) bg = (float)1e-2;
+    }
+    st->bg = bg;
+    // flat start: (float)sum / (float)wMNP_reference  (method_shb.c:255,266)
+    const float sumg = (float)vsum;
+    st->c0 = sumg / (float)J.refMNP;
+    const long long a = (long long)vidx;
+    const long long pMN = (long long)J.pM * J.pN;
+    st->mid[2] = (int)(a / pMN);
+    st->mid[1] = (int)((a - (long long)st->mid[2] * pMN) / J.pM);
+    st->mid[0] = (int)(a - (long long)st->mid[2] * pMN - (long long)st->mid[1] * J.pM);
+    st->iter = 0;
+}
+
+// Start guess into U0 and U1 (method_shb.c:264-287).
+__global__ __launch_bounds__(STHR) void k_start(const JobDev* __restrict__ jobs, int j)
+{
+    const JobDev& J = jobs[j];
+    const size_t n = (size_t)J.nx * J.ny * J.nz;
+    const float c0 = J.st->c0;
+    const float* src = J.start_img ? J.start_img : J.im;
+    for (size_t i = (size_t)blockIdx.x * STHR + threadIdx.x; i < n; i += (size_t)gridDim.x * STHR) {
+        float v = c0;
+        if (J.start_identity) {
+            const int m = (int)(i % J.nx);
+            const size_t r = i / J.nx;
+            const int yy = (int)(r % J.ny), zz = (int)(r / J.ny);
+            v = (m < J.M && yy < J.N && zz < J.P) ? src[m + (size_t)J.M * (yy + (size_t)J.N * zz)] : 0.f;
+        }
+        J.U0[i] = v;
+        J.U1[i] = v;
+    }
+}
+
+// End of a round: error of the iteration from the X-tile partials, in a
+// fixed order, then advance the iteration counter.  One block per job.
+__global__ __launch_bounds__(STHR) void k_round_end(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs,
+                                                    int advance)
+{
+    __shared__ double red[STHR / 64];
+    const JobDev& J = jobs[blockIdx.x];
+    const int nt = J.ntile[KIND_X];
+    double s = 0;
+    for (int i = threadIdx.x; i < nt; i += STHR) s += J.part[i];
+    s = block_sum<STHR>(s, red);
+    if (threadIdx.x == 0) {
+        JobState* st = J.st;
+        const int it = st->iter;
+        const double mean = s / (double)((long long)J.M * J.N * J.P);
+        if (it < J.niter) {
+            J.err[it] = (float)mean;
+            J.errd[it] = mean;
+        }
+        if (advance) st->iter = it + 1;
+    }
+}
+
+// Crop x_k (k = iteration index given) to the image box.
+__global__ __launch_bounds__(STHR) void k_crop(const JobDev* __restrict__ jobs, int j, int k, float* __restrict__ out)
+{
+    const JobDev& J = jobs[j];
+    const float* U = (k & 1) ? J.U1 : J.U0;
+    const size_t n = (size_t)J.M * J.N * J.P;
+    for (size_t i = (size_t)blockIdx.x * STHR + threadIdx.x; i < n; i += (size_t)gridDim.x * STHR) {
+        const int m = (int)(i % J.M);
+        const size_t r = i / J.M;
+        const int yy = (int)(r % J.N), zz = (int)(r / J.N);
+        out[i] = U[m + (size_t)J.nx * (yy + (size_t)J.ny * zz)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers (host)
+// ---------------------------------------------------------------------------
+#define DECL_LAUNCH(name) \
+    extern "C" hipError_t name(const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s);
+DECL_LAUNCH(dwhip_launch_x0)
+DECL_LAUNCH(dwhip_launch_x1)
+DECL_LAUNCH(dwhip_launch_x2)
+DECL_LAUNCH(dwhip_launch_x3)
+DECL_LAUNCH(dwhip_launch_x4)
+DECL_LAUNCH(dwhip_launch_x5)
+DECL_LAUNCH(dwhip_launch_x6)
+DECL_LAUNCH(dwhip_launch_c0)
+DECL_LAUNCH(dwhip_launch_c1)
+DECL_LAUNCH(dwhip_launch_c2)
+DECL_LAUNCH(dwhip_launch_c3)
+DECL_LAUNCH(dwhip_launch_c4)
+DECL_LAUNCH(dwhip_launch_c5)
+DECL_LAUNCH(dwhip_launch_c6)
+
+extern "C" {
+
+hipError_t dwhip_launch_xpass(int mode, const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s)
+{
+    static_assert(XM_PSF == 0 && XM_STORE == 6, "X modes 0..6");
+    switch (mode) {
+    case 0: return dwhip_launch_x0(bd, jobs, grid, lds, s);
+    case 1: return dwhip_launch_x1(bd, jobs, grid, lds, s);
+    case 2: return dwhip_launch_x2(bd, jobs, grid, lds, s);
+    case 3: return dwhip_launch_x3(bd, jobs, grid, lds, s);
+    case 4: return dwhip_launch_x4(bd, jobs, grid, lds, s);
+    case 5: return dwhip_launch_x5(bd, jobs, grid, lds, s);
+    case 6: return dwhip_launch_x6(bd, jobs, grid, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t dwhip_launch_colpass(int mode, const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s)
+{
+    static_assert(CM_YFWD == 0 && CM_ZINV == 6, "column modes 0..6");
+    switch (mode) {
+    case 0: return dwhip_launch_c0(bd, jobs, grid, lds, s);
+    case 1: return dwhip_launch_c1(bd, jobs, grid, lds, s);
+    case 2: return dwhip_launch_c2(bd, jobs, grid, lds, s);
+    case 3: return dwhip_launch_c3(bd, jobs, grid, lds, s);
+    case 4: return dwhip_launch_c4(bd, jobs, grid, lds, s);
+    case 5: return dwhip_launch_c5(bd, jobs, grid, lds, s);
+    case 6: return dwhip_launch_c6(bd, jobs, grid, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t dwhip_launch_stats(const JobDev* jobs, int j, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stats_partial, dim3(nblk), dim3(STHR), 0, s, jobs, j, nblk);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(STHR), 0, s, jobs, j, nblk);
+    return hipGetLastError();
+}
+
+hipError_t dwhip_launch_start(const JobDev* jobs, int j, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_start, dim3(grid), dim3(STHR), 0, s, jobs, j);
+    return hipGetLastError();
+}
+
+hipError_t dwhip_launch_round_end(const BatchDev* bd, const JobDev* jobs, int njobs, int advance, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_round_end, dim3(njobs), dim3(STHR), 0, s, bd, jobs, advance);
+    return hipGetLastError();
+}
+
+hipError_t dwhip_launch_crop(const JobDev* jobs, int j, int k, float* out, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_crop, dim3(grid), dim3(STHR), 0, s, jobs, j, k, out);
+    return hipGetLastError();
+}
+
+}  // extern "C"

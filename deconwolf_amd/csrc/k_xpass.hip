@@ -1,0 +1,20 @@
+// One X-pass mode (XMODE, set by the build) of the SHB loop.
+#include "shb_passes.h"
+
+#ifndef XMODE
+#error "compile with -DXMODE=<mode>"
+#endif
+#define DWHIP_CAT2(a, b) a##b
+#define DWHIP_CAT(a, b) DWHIP_CAT2(a, b)
+
+extern "C" hipError_t DWHIP_CAT(dwhip_launch_x, XMODE)(const BatchDev* bd, const JobDev* jobs, int grid, size_t lds,
+                                                      hipStream_t s)
+{
+    static bool once = false;
+    if (!once) {
+        once = true;
+        dwhip_allow_big_lds(reinterpret_cast<const void*>(k_xpass<XMODE>));
+    }
+    hipLaunchKernelGGL(k_xpass<XMODE>, dim3(grid), dim3(XTHR), lds, s, bd, jobs);
+    return hipGetLastError();
+}

@@ -8,7 +8,7 @@
 // of them.
 #pragma once
 #include <stdint.h>
-#include "fft_lds.h"
+#include "fft_engine.h"
 
 enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2, KIND_N = 3 };
 
@@ -62,8 +62,9 @@ struct JobDev {
     double* stat;       // setup reduction scratch
     float* err;         // per-iteration error, niter floats
     double* errd;       // per-iteration error in double, niter
-    // FFT plans: complex lengths h (x rows), ny, nz; twiddles relative to tw
-    FftPlanDev px, py, pz;
+    // FFT plans for the complex lengths h (x rows), ny and nz: forward with
+    // radices R1..Rk, inverse with Rk..R1; twiddles relative to tw
+    FftPlanDev pxf, pxi, pyf, pyi, pzf, pzi;
     const float2* tw;
     const float2* rtw;  // exp(-2 pi i k / nx), k < h
 };

@@ -10,9 +10,19 @@ enum { XM_PSF = 0, XM_ONES = 1, XM_WEIGHTS = 2, XM_UPDATE = 3, XM_RATIO = 4, XM_
 // Column-pass modes; ZOTF: forward, divide by nx*ny*nz, store into K.
 enum { CM_YFWD = 0, CM_YINV = 1, CM_ZOTF = 2, CM_ZCONV = 3, CM_ZCONVJ = 4, CM_ZFWD = 5, CM_ZINV = 6 };
 
-#define DWHIP_NTHR 256
-#define DWHIP_XMAXE 16   // elements per thread per stage, X pass
-#define DWHIP_CMAXE 40   // elements per thread per stage, Y/Z passes
+#ifndef DWHIP_XTHR
+#define DWHIP_XTHR 256   // threads per X-pass workgroup
+#endif
+#ifndef DWHIP_CTHR
+#define DWHIP_CTHR 512   // threads per column-pass workgroup
+#endif
+#ifndef DWHIP_XMAXE
+#define DWHIP_XMAXE 25   // complex elements per thread per stage, X pass
+#endif
+#ifndef DWHIP_CMAXE
+#define DWHIP_CMAXE 20   // complex elements per thread per stage, Y/Z passes
+#endif
+#define DWHIP_MAX_RADIX 25
 
 extern "C" {
 hipError_t dwhip_launch_xpass(int mode, const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s);

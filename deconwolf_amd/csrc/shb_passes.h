@@ -1,0 +1,384 @@
+// Pass kernels of the SHB (scaled heavy ball Richardson-Lucy) loop for
+// gfx950, as templates; k_xpass.hip and k_colpass.hip instantiate one mode
+// per translation unit.  Restates deconwolf's src/method_shb.c (deconvolve_shb :77-424,
+// iter_shb :10-74) as fused passes over a job table (shb_dev.h).
+//
+// A 3-D R2C/C2R transform of an nx*ny*nz volume is three passes:
+//   X: x-rows, real length nx packed as h = nx/2 complex values, complex
+//      FFT of length h plus the split/merge step to h+1 bins;
+//   Y: columns along y of the (h+1)*ny*nz spectrum, tiles of adjacent kx;
+//   Z: columns along z, tiles of adjacent kx.
+// One SHB iteration k runs eight passes (A..H):
+//   A  X: [C2R of the previous adjoint -> x_k = xt*p_{k-1}*W, clamp]
+//         p_k = max(x_k + a_k (x_k - x_{k-1}), bg); R2C(p_k)
+//   B  Y forward
+//   C  Z forward, * OTF, Z inverse
+//   D  Y inverse
+//   E  X: C2R -> y = H p_k; error terms; ratio im/y; R2C(ratio)
+//   F  Y forward
+//   G  Z forward, * conj(OTF), Z inverse
+//   H  Y inverse
+// and the next iteration's A finishes the update.  The 1/(nx*ny*nz) of both
+// inverse transforms is folded into the stored OTF.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "fft_engine.h"
+#include "shb_dev.h"
+#include "shb_kernels.h"
+
+#define XTHR DWHIP_XTHR
+#define CTHR DWHIP_CTHR
+
+__device__ __forceinline__ int find_job(const JobDev* __restrict__ jobs, int njobs, int kind, int t)
+{
+    int j = 0;
+    while (j + 1 < njobs && t >= jobs[j + 1].tile0[kind]) ++j;
+    return j;
+}
+
+// alpha of method_shb.c:328-330, in double like the reference.
+__device__ __forceinline__ double shb_alpha(int iter, float alphamax)
+{
+    double a = ((float)iter - 1.0) / ((float)iter + 2.0);
+    if (a < 0) a = 0;
+    if (a > alphamax) a = alphamax;
+    return a;
+}
+
+// p = x + alpha*(x - xp), then p = max(p, bg)  (method_shb.c:333-339):
+// x - xp in float, the rest in double, rounded to float.
+__device__ __forceinline__ float shb_extrap(float x, float xp, double alpha, float bg)
+{
+    float p = (float)((double)x + alpha * (double)(x - xp));
+    return p < bg ? bg : p;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NT / 64; ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// X pass
+// ---------------------------------------------------------------------------
+// C2R split step: builds Z[k] (length h) from the h+1 bins of a row so that
+// IFFT_h(Z)[n] = out[2n] + i*out[2n+1] of the unnormalised C2R.
+//   Z[k] = (X[k] + conj(X[h-k])) + i*exp(+2 pi i k/nx)*(X[k] - conj(X[h-k]))
+__device__ __forceinline__ float2 c2r_split(float2 a, float2 b, float2 w)
+{
+    // a = X[k], b = X[h-k], w = exp(-2 pi i k/nx)
+    float2 s = make_float2(a.x + b.x, a.y - b.y);   // a + conj(b)
+    float2 d = make_float2(a.x - b.x, a.y + b.y);   // a - conj(b)
+    float2 e = cmulc(d, w);                          // d * exp(+2 pi i k/nx)
+    return make_float2(s.x - e.y, s.y + e.x);        // s + i*e
+}
+
+// R2C merge step: X[k] from Z[k], Z[h-k] of the packed length-h FFT.
+//   X[k] = (Z[k] + conj(Z[h-k]))/2 - i/2 * exp(-2 pi i k/nx) * (Z[k] - conj(Z[h-k]))
+__device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
+{
+    float2 s = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+    float2 d = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y));
+    float2 e = cmul(d, w);
+    return make_float2(s.x + e.y, s.y - e.x);        // s - i*e
+}
+
+#define XCHUNK 8  // elements per thread whose loads are issued together
+
+template <int MODE>
+__global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
+{
+    extern __shared__ float2 lds[];
+    __shared__ double red[XTHR / 64];
+    const int tid = threadIdx.x;
+    const int total = bd->ntile[KIND_X];
+    const int njobs = bd->njobs;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const JobDev& J = jobs[find_job(jobs, njobs, KIND_X, t)];
+        const int lt = t - J.tile0[KIND_X];
+        const int h = J.h, nx = J.nx, ny = J.ny, hcs = J.hcs;
+        const int row0 = lt * J.xrows;
+        const int nrows = min(J.xrows, ny * J.nz - row0);
+        const int iter = (MODE == XM_UPDATE || MODE == XM_RATIO) ? J.st->iter : 0;
+        const bool inv = MODE == XM_RATIO || MODE == XM_WEIGHTS || MODE == XM_STORE ||
+                         (MODE == XM_UPDATE && iter > 0);
+        const bool fwd = MODE != XM_WEIGHTS && MODE != XM_STORE;
+        StageGeo g;
+        g.n = h;
+        g.npen = nrows;
+        g.pstride = h;
+        g.nthr = XTHR;
+        g.npmagic = fdiv_magic((uint32_t)nrows);
+        const uint32_t hmagic = fdiv_magic((uint32_t)h);
+        const float2* __restrict__ C = J.C;
+
+        auto lds_src = [&](int pen, int i) { return lds[pen * h + i]; };
+        auto lds_dst = [&](int pen, int i, float2 v) { lds[pen * h + i] = v; };
+
+        if (inv) {
+            auto split_src = [&](int pen, int i) {
+                const float2* crow = C + (size_t)(row0 + pen) * hcs;
+                return c2r_split(crow[i], crow[h - i], J.rtw[i]);
+            };
+            fft_tile<DWHIP_XMAXE, +1, false, false, true>(J.pxi, g, tid, lds, J.tw, split_src, lds_dst);
+            __syncthreads();
+        }
+
+        // elementwise step on packed real pairs (elements 2n, 2n+1 of a row),
+        // in natural order in LDS; loads of XCHUNK elements are issued together
+        double acc = 0;
+        const int nwork = nrows * h;
+        float* Uk = (iter & 1) ? J.U1 : J.U0;          // x_{k-2} in, x_k out (k > 0)
+        const float* Ukm1 = (iter & 1) ? J.U0 : J.U1;  // x_{k-1}
+        const float bg = J.st->bg;
+        for (int w0 = tid; w0 < nwork; w0 += XCHUNK * XTHR) {
+            float2 a[XCHUNK], b[XCHUNK], c[XCHUNK], wv[XCHUNK];
+            size_t vi[XCHUNK];
+            int yy[XCHUNK], zz[XCHUNK], m0[XCHUNK];
+#pragma unroll
+            for (int q = 0; q < XCHUNK; ++q) {
+                const int w = min(w0 + q * XTHR, nwork - 1);
+                const int bb = (int)fdiv((uint32_t)w, hmagic);
+                const int n = w - bb * h;
+                const int row = row0 + bb;
+                yy[q] = row % ny;
+                zz[q] = row / ny;
+                m0[q] = 2 * n;
+                vi[q] = (size_t)row * nx + 2 * n;
+                a[q] = b[q] = c[q] = wv[q] = make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int q = 0; q < XCHUNK; ++q) {
+                if (w0 + q * XTHR >= nwork) continue;
+                if (inv) a[q] = lds[w0 + q * XTHR];
+                if (MODE == XM_UPDATE) {
+                    b[q] = *reinterpret_cast<const float2*>(Ukm1 + vi[q]);
+                    c[q] = *reinterpret_cast<const float2*>(Uk + vi[q]);
+                    if (iter > 0 && J.has_w) wv[q] = *reinterpret_cast<const float2*>(J.W + vi[q]);
+                } else if (MODE == XM_LOAD) {
+                    a[q] = *reinterpret_cast<const float2*>(J.U0 + vi[q]);
+                } else if (MODE == XM_RATIO) {
+                    if (yy[q] < J.N && zz[q] < J.P) {
+                        const float* imrow = J.im + (size_t)J.M * (yy[q] + (size_t)J.N * zz[q]);
+                        if (m0[q] < J.M) b[q].x = imrow[m0[q]];
+                        if (m0[q] + 1 < J.M) b[q].y = imrow[m0[q] + 1];
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < XCHUNK; ++q) {
+                const int w = w0 + q * XTHR;
+                if (w >= nwork) continue;
+                float2 out;
+                if (MODE == XM_PSF) {
+                    const JobState* st = J.st;
+                    const int sy = (yy[q] + st->mid[1]) % ny, sz = (zz[q] + st->mid[2]) % J.nz;
+                    float v[2] = {0.f, 0.f};
+                    if (sy < J.pN && sz < J.pP) {
+#pragma unroll
+                        for (int e = 0; e < 2; ++e) {
+                            const int sx = (m0[q] + e + st->mid[0]) % nx;
+                            if (sx < J.pM) v[e] = J.psf[sx + (size_t)J.pM * (sy + (size_t)J.pN * sz)];
+                        }
+                    }
+                    out = make_float2(v[0], v[1]);
+                } else if (MODE == XM_LOAD) {
+                    out = a[q];
+                } else if (MODE == XM_STORE) {
+                    out = a[q];
+                    *reinterpret_cast<float2*>(J.U0 + vi[q]) = out;
+                } else if (MODE == XM_ONES) {
+                    const bool inr = yy[q] < J.N && zz[q] < J.P;
+                    out = make_float2(inr && m0[q] < J.M ? 1.f : 0.f, inr && m0[q] + 1 < J.M ? 1.f : 0.f);
+                } else if (MODE == XM_WEIGHTS) {
+                    // method_shb.c:225-234: W = P1 > 0.01 ? 1/P1 : 0
+                    const float2 p1 = a[q];
+                    const float sigma = 0.01f;
+                    out.x = p1.x > sigma ? (float)(1.0 / p1.x) : 0.f;
+                    out.y = p1.y > sigma ? (float)(1.0 / p1.y) : 0.f;
+                    *reinterpret_cast<float2*>(J.W + vi[q]) = out;
+                } else if (MODE == XM_UPDATE) {
+                    const float2 xkm1 = b[q];
+                    float2 xk;
+                    if (iter == 0) {
+                        xk = c[q];  // U0 = x_0, U1 = x_{-1} = x_0 (start guess)
+                    } else {
+                        // finish iteration k-1 (method_shb.c:55-70, 357-376)
+                        const float2 xt = a[q];
+                        const float2 xkm2 = c[q];
+                        const double am1 = shb_alpha(iter - 1, J.alphamax);
+                        const float p0 = shb_extrap(xkm1.x, xkm2.x, am1, bg);
+                        const float p1 = shb_extrap(xkm1.y, xkm2.y, am1, bg);
+                        if (J.has_w)
+                            xk = make_float2(xt.x * (p0 * wv[q].x), xt.y * (p1 * wv[q].y));
+                        else
+                            xk = make_float2(xt.x * p0, xt.y * p1);
+                        if (J.positivity) {
+                            xk.x = xk.x < bg ? bg : xk.x;
+                            xk.y = xk.y < bg ? bg : xk.y;
+                        }
+                        *reinterpret_cast<float2*>(Uk + vi[q]) = xk;
+                    }
+                    const double al = shb_alpha(iter, J.alphamax);
+                    out = make_float2(shb_extrap(xk.x, xkm1.x, al, bg), shb_extrap(xk.y, xkm1.y, al, bg));
+                } else {  // XM_RATIO, method_shb.c:27-48 and getError dw.c:1436-1524
+                    const float2 yv = a[q];
+                    float r[2] = {0.f, 0.f};
+                    if (yy[q] < J.N && zz[q] < J.P) {
+#pragma unroll
+                        for (int e = 0; e < 2; ++e) {
+                            const int m = m0[q] + e;
+                            if (m < J.M) {
+                                float y = e ? yv.y : yv.x;
+                                const float gv = e ? b[q].y : b[q].x;
+                                const double yd = y, gd = gv;
+                                if (J.metric == DWHIP_DEV_METRIC_IDIV) {
+                                    if (yd > 0 && gd > 0) acc += gd * (double)logf((float)(gd / yd)) - (gd - yd);
+                                } else {
+                                    acc += (yd - gd) * (yd - gd);
+                                }
+                                if (fabs(yd) < 1e-6) y = (float)copysign(1e-6, yd);
+                                r[e] = gv / y;
+                            }
+                        }
+                    }
+                    out = make_float2(r[0], r[1]);
+                }
+                lds[w] = out;
+            }
+        }
+        if (MODE == XM_RATIO) {
+            const double s = block_sum<XTHR>(acc, red);
+            if (tid == 0) J.part[lt] = s;
+        }
+        __syncthreads();
+
+        if (fwd) {
+            fft_tile<DWHIP_XMAXE, -1, false, true, true>(J.pxf, g, tid, lds, J.tw, lds_src, lds_dst);
+            __syncthreads();
+            const int hp = h / 2 + 1;
+            const uint32_t hpmagic = fdiv_magic((uint32_t)hp);
+            float2* __restrict__ dst = J.C;
+            for (int w = tid; w < nrows * hp; w += XTHR) {
+                const int b = (int)fdiv((uint32_t)w, hpmagic);
+                const int k = w - b * hp;
+                const float2* L = lds + b * h;
+                float2* crow = dst + (size_t)(row0 + b) * hcs;
+                if (k == 0) {
+                    const float2 z0 = L[0];
+                    crow[0] = make_float2(z0.x + z0.y, 0.f);
+                    crow[h] = make_float2(z0.x - z0.y, 0.f);
+                } else {
+                    const float2 a = L[k], c = L[h - k];
+                    crow[k] = r2c_merge(a, c, J.rtw[k]);
+                    if (2 * k != h) crow[h - k] = r2c_merge(c, a, J.rtw[h - k]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column passes (Y and Z)
+// ---------------------------------------------------------------------------
+// OTF multiply at the Z junction: fft_mul_inplace (fft.c:437-453) and
+// fft_mul_conj_inplace (fft.c:490-512) with their operand order.
+template <bool CONJ>
+struct OtfOp {
+    const float2* __restrict__ K;
+    size_t stride;
+    int nb;
+    __device__ __forceinline__ float2 fetch(int pen, int i) const
+    {
+        return pen < nb ? K[i * stride + pen] : make_float2(0.f, 0.f);
+    }
+    __device__ __forceinline__ float2 apply(float2 v, float2 kk) const
+    {
+        const float a = kk.x, ac = CONJ ? -kk.y : kk.y;
+        return make_float2(a * v.x - ac * v.y, a * v.y + v.x * ac);
+    }
+};
+
+template <int MODE>
+__global__ __launch_bounds__(CTHR) void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
+{
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x;
+    constexpr int kind = (MODE == CM_YFWD || MODE == CM_YINV) ? KIND_Y : KIND_Z;
+    static_assert(MODE >= CM_YFWD && MODE <= CM_ZINV, "column mode");
+    const int total = bd->ntile[kind];
+    const int njobs = bd->njobs;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
+        const int lt = t - J.tile0[kind];
+        const int hcs = J.hcs, ny = J.ny;
+        const int B = kind == KIND_Y ? J.ycols : J.zcols;
+        const int nct = (J.hc + B - 1) / B;
+        const int outer = lt / nct;
+        const int kx0 = (lt - outer * nct) * B;
+        const int nb = min(B, J.hc - kx0);   // columns past hc are row padding, never touched
+        const size_t stride = kind == KIND_Y ? (size_t)hcs : (size_t)hcs * ny;
+        const size_t off = kind == KIND_Y ? (size_t)outer * ny * hcs + kx0 : (size_t)outer * hcs + kx0;
+        const float2* __restrict__ src = J.C + off;
+        float2* __restrict__ dst = (MODE == CM_ZOTF ? J.K : J.C) + off;
+        StageGeo g;
+        g.n = kind == KIND_Y ? ny : J.nz;
+        g.npen = B;
+        g.pstride = 0;
+        g.nthr = CTHR;
+        g.npmagic = fdiv_magic((uint32_t)B);
+
+        auto gsrc = [&](int pen, int i) { return pen < nb ? src[i * stride + pen] : make_float2(0.f, 0.f); };
+        auto gdst = [&](int pen, int i, float2 v) {
+            if (pen < nb) dst[i * stride + pen] = v;
+        };
+        if (MODE == CM_YFWD) {
+            fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pyf, g, tid, lds, J.tw, gsrc, gdst);
+        } else if (MODE == CM_YINV) {
+            fft_tile<DWHIP_CMAXE, +1, true, false, false>(J.pyi, g, tid, lds, J.tw, gsrc, gdst);
+        } else if (MODE == CM_ZFWD) {
+            fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pzf, g, tid, lds, J.tw, gsrc, gdst);
+        } else if (MODE == CM_ZINV) {
+            fft_tile<DWHIP_CMAXE, +1, true, false, false>(J.pzi, g, tid, lds, J.tw, gsrc, gdst);
+        } else if (MODE == CM_ZOTF) {
+            // (float)nx*ny*nz, evaluated left to right in float (fft.c:856)
+            const float scale = ((float)J.nx * (float)J.ny) * (float)J.nz;
+            auto sdst = [&](int pen, int i, float2 v) {
+                if (pen < nb) dst[i * stride + pen] = make_float2(v.x / scale, v.y / scale);
+            };
+            fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pzf, g, tid, lds, J.tw, gsrc, sdst);
+        } else {
+            OtfOp<MODE == CM_ZCONVJ> op{J.K + off, stride, nb};
+            fft_pair_tile<DWHIP_CMAXE, -1, +1, true>(J.pzf, J.pzi, g, tid, lds, J.tw, gsrc, op, gdst);
+        }
+        __syncthreads();
+    }
+}
+
+
+// Dynamic LDS above 64 KiB must be allowed per kernel (gfx950 has 160 KiB
+// per workgroup).  The limit asked for leaves 1 KiB for static LDS.
+static inline void dwhip_allow_big_lds(const void* kernel)
+{
+    int v = 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess) v = 65536;
+    hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, v - 1024);
+    if (e != hipSuccess) {
+        fprintf(stderr, "dwhip: raising the LDS limit to %d bytes failed: %s\n", v - 1024, hipGetErrorString(e));
+        (void)hipGetLastError();
+    }
+}

@@ -43,7 +43,13 @@ def case(shape, psfshape, seed=1, npts=30, bg=50.0, psf="gauss"):
     return im, k
 
 
-@pytest.mark.parametrize("shape", [(10, 12, 16), (15, 21, 70), (32, 64, 128), (14, 243, 250), (9, 5, 2)])
+# (nz, ny, nx): cover every radix of the planner (2..9, 15, 16, 20, 25),
+# one to four stages, n = 1, and the split step for odd and even h = nx/2
+FFT_SHAPES = [(10, 12, 16), (15, 21, 70), (32, 64, 128), (14, 243, 250), (9, 5, 2), (16, 256, 1250),
+              (140, 36, 40), (162, 30, 48), (96, 1215, 36), (4, 2401, 6), (1, 1, 8)]
+
+
+@pytest.mark.parametrize("shape", FFT_SHAPES)
 def test_fft3_forward_inverse(shape):
     print("fft3", shape, flush=True)
     rng = np.random.default_rng(7)
