@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""SHB deconvolution benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu-baseline]
+
+A step is one complete deconvolution job (set-up plus all SHB iterations)
+of the workload, with the image and PSF already resident in GPU memory; K
+steps are timed.  The workload (deconwolf_amd/workloads.py) is made with
+numpy/scipy before the GPU is touched, the CPU baseline (the oracle's
+float32 SHB with MKL's FFTW3-API transforms) runs on it in a child process,
+then the GPU runs.
+
+Prints one JSON line.  `value` is image voxel-iterations per second
+(M*N*P*iterations per second of wall time over the timed steps).
+`roofline` is for the kernel with the largest total time: its algorithmic
+bytes per launch over its mean launch duration, both from HIP events
+around every launch on the stream it runs on.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SHB voxel-iter/s batched-served vs plain per-image; p50/p99 job latency"
+
+
+def cpu_threads():
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def run_cpu_baseline(npz, iters_full, threads, timeout_s):
+    env = dict(os.environ)
+    env["MKL_THREADING_LAYER"] = "GNU"
+    env["OMP_NUM_THREADS"] = str(threads)
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--inputs", npz, "--iters", "2",
+           "--full-iters", str(iters_full), "--threads", str(threads)]
+    t = time.time()
+    try:
+        r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"value": None, "unit": "voxel-iter/s", "cores": threads, "kind": "port",
+                "sample": "timed out after %d s" % timeout_s}
+    if r.returncode != 0:
+        return {"value": None, "unit": "voxel-iter/s", "cores": threads, "kind": "port",
+                "sample": "failed: " + r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "failed"}
+    line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
+    d = json.loads(line)
+    d["wall_s"] = time.time() - t
+    return d
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-timeout", type=int, default=240)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (jobs would shard one per GPU "
+              "with no communication); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from deconwolf_amd import workloads
+
+    t0 = time.time()
+    im, psf, s, raw_psf = workloads.make(a.config)
+    t_gen = time.time() - t0
+    P, N, M = im.shape
+    iters = s.nIter
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        with tempfile.TemporaryDirectory() as td:
+            npz = os.path.join(td, "inputs.npz")
+            np.savez(npz, im=im, psf=psf)
+            cpu = run_cpu_baseline(npz, iters, cpu_threads(), a.cpu_timeout)
+
+    import torch
+    import deconwolf_amd as dw
+    from deconwolf_amd import _lib
+
+    dev = torch.device("cuda:0")
+    dim = torch.from_numpy(im).to(dev)
+    dpsf = torch.from_numpy(psf).to(dev)
+    out = torch.empty_like(dim)
+    torch.cuda.synchronize()
+
+    def step():
+        st = dw.DwOpts(nIter=iters, iter_type=dw.ITER_FIXED)
+        dw.deconvolve_shb_hip_device(dim, dpsf, st, out=out)
+        return st
+
+    for _ in range(a.warmup):
+        st = step()
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    lat = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        t1 = time.perf_counter()
+        st = step()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+
+    value = M * N * P * iters * a.steps / elapsed
+    kernels = {}
+    for name, (n, ms, by) in prof.items():
+        kernels[name] = {"launches": n, "avg_us": 1e3 * ms / n, "bytes_per_launch": by / n,
+                         "GBps": (by / 1e9) / (ms / 1e3)}
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dname, (dn, dms, dby) = dom
+    achieved = (dby / dn / 1e9) / (dms / dn / 1e3)
+    all_ms = sum(v[1] for v in prof.values())
+    all_by = sum(v[2] for v in prof.values())
+    wx, wy, wz = st.stats["work"]
+    lat_sorted = sorted(lat)
+
+    def pct(p):
+        return lat_sorted[min(len(lat_sorted) - 1, int(round(p / 100.0 * (len(lat_sorted) - 1))))]
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-iter/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%s: %s" % (a.config, workloads.describe(a.config)),
+            "path": "plain per-image (dwhip_shb_run_device, one job at a time)",
+            "iterations": iters,
+            "image": [M, N, P],
+            "psf": [psf.shape[2], psf.shape[1], psf.shape[0]],
+            "job": [wx, wy, wz],
+            "reference_job": list(st.stats["ref_work"]),
+        },
+        "p50_job_ms": pct(50),
+        "p99_job_ms": pct(99),
+        "roofline": {
+            "kernel": dname,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": dby / dn,
+            "avg_launch_us": 1e3 * dms / dn,
+            "all_passes_GBps": (all_by / 1e9) / (all_ms / 1e3),
+        },
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "input_gen_s": t_gen,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

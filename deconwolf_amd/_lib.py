@@ -78,7 +78,30 @@ SIGNATURES = [
     ("dwhip_shb_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                                     ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
     ("dwhip_fft3_device", _int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
+    ("dwhip_profile_enable", _int, [_int]),
+    ("dwhip_profile_get", _int, [_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 ]
+
+N_PROFILE_KERNELS = 14
+
+
+def profile_enable(on=True):
+    check(lib().dwhip_profile_enable(1 if on else 0), "dwhip_profile_enable")
+
+
+def profile_read():
+    """{kernel name: (launches, total_ms, algorithmic bytes)} for kernels that ran."""
+    out = {}
+    for k in range(N_PROFILE_KERNELS):
+        name = ctypes.c_char_p()
+        n = ctypes.c_longlong()
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        check(lib().dwhip_profile_get(k, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)))
+        if n.value:
+            out[name.value.decode()] = (int(n.value), float(ms.value), float(by.value))
+    return out
 
 _lib = None
 

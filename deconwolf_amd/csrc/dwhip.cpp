@@ -411,22 +411,51 @@ int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int
     return DWHIP_OK;
 }
 
+// Algorithmic HBM bytes of one pass over job d: each operand read or
+// written once, complex volumes counted at (nx/2+1)*ny*nz values (row
+// padding excluded).  xkind: 1 for X passes, 0 for column passes; iter
+// matters for the update pass (iteration 0 has no inverse transform).
+double dwhip_pass_bytes(int xkind, int mode, const JobDev& d, int iter)
+{
+    const double Vc = (double)d.hc * d.ny * d.nz * sizeof(float2);
+    const double V = (double)d.nx * d.ny * d.nz * sizeof(float);
+    const double img = (double)d.M * d.N * d.P * sizeof(float);
+    if (!xkind) return (mode == CM_ZCONV || mode == CM_ZCONVJ) ? 3 * Vc : 2 * Vc;
+    switch (mode) {
+    case XM_PSF: return Vc + (double)d.pM * d.pN * d.pP * sizeof(float);
+    case XM_ONES: return Vc;
+    case XM_WEIGHTS: return Vc + V;
+    case XM_UPDATE: return iter == 0 ? 2 * V + Vc : 2 * Vc + (d.has_w ? 4 : 3) * V;
+    case XM_RATIO: return 2 * Vc + img;
+    case XM_LOAD: return Vc + V;
+    case XM_STORE: return Vc + V;
+    default: return 0;
+    }
+}
+
+static double host_iter(const DwhipJob* j, int mode)
+{
+    return mode == XM_UPDATE ? (double)j->host_iter : 0;
+}
+
 int dwhip_pass_x(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* jd, int ntile, size_t lds,
                  hipStream_t s)
 {
     if (ntile <= 0) return DWHIP_OK;
+    const int slot = dwhip_prof_begin(s);
     hipError_t e = dwhip_launch_xpass(mode, bd, jd, ntile, lds, s);
+    if (slot >= 0) dwhip_prof_end(s, slot, mode, dwhip_pass_bytes(1, mode, j->d, (int)host_iter(j, mode)));
     return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "x pass", __LINE__);
-    (void)j;
 }
 
 int dwhip_pass_c(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* jd, int ntile, size_t lds,
                  hipStream_t s)
 {
     if (ntile <= 0) return DWHIP_OK;
+    const int slot = dwhip_prof_begin(s);
     hipError_t e = dwhip_launch_colpass(mode, bd, jd, ntile, lds, s);
+    if (slot >= 0) dwhip_prof_end(s, slot, 7 + mode, dwhip_pass_bytes(0, mode, j->d, 0));
     return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "column pass", __LINE__);
-    (void)j;
 }
 
 #define RCHK(call)                      \
@@ -459,7 +488,7 @@ int dwhip_job_setup(const DwhipJob* j, hipStream_t s)
 
 // One round (iteration k) of a single job; `full` = also run the adjoint
 // half (F, G, H), which only the next iteration's update consumes.
-int dwhip_job_round(const DwhipJob* j, bool full, hipStream_t s)
+int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s)
 {
     const DwhipTiles& t = j->tiles;
     RCHK(dwhip_pass_x(XM_UPDATE, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
@@ -473,6 +502,7 @@ int dwhip_job_round(const DwhipJob* j, bool full, hipStream_t s)
         RCHK(dwhip_pass_c(CM_YINV, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
     }
     HIPCHK(dwhip_launch_round_end(j->bd, j->jd, 1, 1, s));
+    j->host_iter += 1;
     return DWHIP_OK;
 }
 
@@ -541,6 +571,7 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
     }
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess && rc == DWHIP_OK) rc = dwhip_hip_fail(e, "synchronize", __LINE__);
+    if (dwhip_prof_on()) dwhip_prof_collect();
     if (rc == DWHIP_OK && st) {
         JobState hs;
         hipMemcpy(&hs, job.d.st, sizeof(hs), hipMemcpyDeviceToHost);

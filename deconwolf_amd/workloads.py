@@ -1,0 +1,74 @@
+"""Synthetic benchmark workloads (seeded), prepared the way deconwolf's
+dw_run prepares a method call (src/dw.c:2799-2852): PSF normalised to sum
+1, autocropped, normalised again; the offset added to the image.
+
+    c1  128x128x32, 40 point sources U(500,5000), background 100,
+        Born-Wolf PSF 127x127x63 (NA 1.40, ni 1.515, 460 nm, 65/200 nm),
+        Poisson noise, 20 iterations
+    c2  1024x1024x60, 2000 Gaussian blobs (sigma U(1,3) px, amplitude
+        U(200,4000)), background 120, Born-Wolf PSF 181x181x79 (520 nm),
+        Poisson noise, 50 iterations
+
+Only numpy and scipy are used (no GPU), so a CPU-only process can make
+the same inputs.
+"""
+import os
+
+import numpy as np
+
+from . import psf_bw, synth
+from .method_shb import DwOpts, fim_normalize_sum1, psf_autocrop
+
+CONFIGS = {
+    "c1": dict(shape=(32, 128, 128), kind="points", n=40, amp=(500.0, 5000.0), bg=100.0,
+               psf=(127, 127, 63), lambda_nm=460.0, dz=200.0, iters=20, seed=1),
+    "c2": dict(shape=(60, 1024, 1024), kind="blobs", n=2000, sigma=(1.0, 3.0), amp=(200.0, 4000.0),
+               bg=120.0, psf=(181, 181, 79), lambda_nm=520.0, dz=200.0, iters=50, seed=2),
+}
+
+
+def describe(name):
+    c = CONFIGS[name]
+    P, N, M = c["shape"]
+    pM, pN, pP = c["psf"]
+    obj = ("%d point sources" % c["n"]) if c["kind"] == "points" else ("%d Gaussian blobs" % c["n"])
+    return ("%dx%dx%d float32, %s, background %g, Born-Wolf PSF %dx%dx%d (NA 1.40, ni 1.515, %g nm, "
+            "dx 65 nm, dz %g nm), Poisson noise, %d SHB iterations"
+            % (M, N, P, obj, c["bg"], pM, pN, pP, c["lambda_nm"], c["dz"], c["iters"]))
+
+
+def _blur_scipy(V, psf):
+    """'same' linear convolution in float32 through scipy.fft (threads)."""
+    from scipy import fft as sfft
+    P, N, M = V.shape
+    pP, pN, pM = psf.shape
+    full = (P + pP - 1, N + pN - 1, M + pM - 1)
+    w = os.cpu_count() or 1
+    a = sfft.rfftn(V.astype(np.float32), full, workers=w)
+    a *= sfft.rfftn(psf.astype(np.float32), full, workers=w)
+    out = sfft.irfftn(a, full, workers=w)
+    cz, cy, cx = (pP - 1) // 2, (pN - 1) // 2, (pM - 1) // 2
+    return out[cz:cz + P, cy:cy + N, cx:cx + M]
+
+
+def make(name):
+    """Returns (im, psf, opts, raw_psf_shape): im with the offset added,
+    psf normalised and autocropped, opts a DwOpts with the iteration count."""
+    c = CONFIGS[name]
+    rng = np.random.default_rng(c["seed"])
+    pM, pN, pP = c["psf"]
+    psf = psf_bw.bw_psf(pM, pN, pP, c["lambda_nm"], NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=c["dz"])
+    if c["kind"] == "points":
+        V = synth.point_sources(c["shape"], c["n"], amp=c["amp"], rng=rng)
+    else:
+        V = synth.gaussian_blobs(c["shape"], c["n"], sigma=c["sigma"], amp=c["amp"], rng=rng)
+    B = _blur_scipy(V, psf)
+    del V
+    im = synth.noisy_image(B, c["bg"], rng=rng)
+    del B
+    s = DwOpts(nIter=c["iters"], iter_type=2)
+    ps = fim_normalize_sum1(psf)
+    ps = psf_autocrop(ps, im.shape, s)
+    ps = fim_normalize_sum1(ps)
+    im += np.float32(s.offset)
+    return np.ascontiguousarray(im), np.ascontiguousarray(ps), s, psf.shape

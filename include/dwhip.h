@@ -143,6 +143,20 @@ int dwhip_shb_run_device(const float *im, int64_t M, int64_t N, int64_t P, const
 int dwhip_fft3_device(float *vol, float *spec, int64_t nx, int64_t ny, int64_t nz, int inverse,
                       void *stream);
 
+/*
+ * Kernel timing for measurement.  While enabled, every pass launch is
+ * bracketed by HIP events on the stream it runs on and its algorithmic
+ * bytes (each operand read or written once) are added up per kernel.
+ * dwhip_profile_enable(1) clears the totals.  Kernel ids 0..13 name the
+ * X-pass modes (x_psf, x_ones, x_weights, x_update, x_ratio, x_load,
+ * x_store) and column modes (y_fwd, y_inv, z_otf, z_conv, z_convj, z_fwd,
+ * z_inv).  No counterpart in deconwolf (its timing is the --times flag,
+ * src/dw.c:2233-2319).
+ */
+int dwhip_profile_enable(int on);
+int dwhip_profile_get(int kernel, const char **name, long long *launches, double *total_ms,
+                      double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

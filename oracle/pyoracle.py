@@ -71,12 +71,21 @@ def lib():
                           ctypes.POINTER(ShbOpts), dp, dp, dp, dp]
             f.restype = ctypes.c_int
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_fft_backend.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.oracle_set_fft_backend.restype = ctypes.c_int
         _lib = L
     return _lib
 
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_fft_backend(external, nthreads=0):
+    """external=True: float transforms through MKL's FFTW3 interface when
+    libmkl_rt.so loads (needs MKL_THREADING_LAYER=GNU in the environment
+    before the process starts); returns whether it is in use."""
+    return bool(lib().oracle_set_fft_backend(1 if external else 0, int(nthreads)))
 
 
 def cfft(x, sign=-1):
@@ -137,7 +146,7 @@ def shb(im, psf, opts=None, dtype=np.float32, start_image=None, **kw):
     out = np.empty_like(im_a)
     nmax = max(o.niter, o.maxiter, 1)
     err = np.zeros(nmax + 1, dtype=np.float32)
-    stats = np.zeros(2, dtype=np.float64)
+    stats = np.zeros(4, dtype=np.float64)
     f = lib().oracle_shb_f32 if dtype == np.float32 else lib().oracle_shb_f64
     rc = f(_ptr(im_a), M, N, P, _ptr(psf_a), pM, pN, pP, ctypes.byref(o), _ptr(out),
            _ptr(err), _ptr(stats))
@@ -145,7 +154,13 @@ def shb(im, psf, opts=None, dtype=np.float32, start_image=None, **kw):
     if rc != 0:
         raise RuntimeError("oracle_shb failed with code %d" % rc)
     n = int(stats[1])
+    global last_times
+    last_times = (float(stats[2]), float(stats[3]))
     return out, err[:n].copy(), float(stats[0]), n
+
+
+# (setup seconds, iteration seconds) of the last shb() call
+last_times = (0.0, 0.0)
 
 
 def work_size(bq, imshape, psfshape):
