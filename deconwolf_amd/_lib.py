@@ -53,6 +53,10 @@ class DwhipStats(ctypes.Structure):
     ]
 
 
+class EngineConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+
+
 class DwhipError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -78,6 +82,14 @@ SIGNATURES = [
     ("dwhip_shb_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                                     ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
     ("dwhip_fft3_device", _int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
+    ("dwhip_engine_config_default", None, [ctypes.POINTER(EngineConfig)]),
+    ("dwhip_engine_create", _int, [ctypes.POINTER(EngineConfig), _vp, ctypes.POINTER(_vp)]),
+    ("dwhip_engine_destroy", None, [_vp]),
+    ("dwhip_engine_submit", _int, [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                                   ctypes.POINTER(DwhipParams), _vp, _i64]),
+    ("dwhip_engine_step", _int, [_vp, _P64, _int, ctypes.POINTER(_int)]),
+    ("dwhip_engine_status", _int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_size_t)]),
+    ("dwhip_engine_result", _int, [_vp, _i64, ctypes.POINTER(DwhipStats), _int]),
     ("dwhip_profile_enable", _int, [_int]),
     ("dwhip_profile_get", _int, [_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_longlong),
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),

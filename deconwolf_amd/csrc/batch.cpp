@@ -1,0 +1,457 @@
+// Batched and served SHB: many deconvolution jobs of different shapes run
+// together.  Each round runs one SHB iteration of every active job, one
+// launch per pass over a table of all of them (the pass kernels walk the
+// jobs' tiles); jobs are admitted from a FIFO queue while their device
+// block fits in the engine's HBM budget, and leave as soon as they finish,
+// so new requests join between rounds (continuous batching).  A job's
+// arithmetic does not depend on the other jobs in its tables: results
+// are the same alone and in the mix.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "dwhip.h"
+#include "dwhip_internal.h"
+#include "shb_kernels.h"
+
+namespace {
+
+#define ECHK(call)                                                        \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return dwhip_hip_fail(e_, #call, __LINE__); \
+    } while (0)
+#define RCHK(call)                     \
+    do {                               \
+        int r_ = (call);               \
+        if (r_ != DWHIP_OK) return r_; \
+    } while (0)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// First-fit allocator over [0, size) with coalescing.
+struct Arena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    size_t used = 0;
+    void init(size_t n)
+    {
+        size = n;
+        free_.clear();
+        free_[0] = n;
+        used = 0;
+    }
+    bool alloc(size_t n, size_t* off)
+    {
+        n = align_up(n, 4096);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                *off = it->first;
+                const size_t rest = it->second - n;
+                const size_t o = it->first + n;
+                free_.erase(it);
+                if (rest) free_[o] = rest;
+                used += n;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = align_up(n, 4096);
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct EngJob {
+    DwhipJob j;
+    dwhip_params prm;
+    int64_t tag = 0;
+    float* out = nullptr;
+    size_t off = 0;
+    bool stop = false;       // REL/ABS: the iterator said stop
+    int nrun = 0;            // iterations run
+    float last_err = 1, prev_err = 1;
+    std::vector<float> err;  // error trace, filled at completion
+    float bg = 0;
+};
+
+// A device table: BatchDev at the start, the JobDev array 64 bytes in.
+struct Table {
+    char* dev = nullptr;
+    char* host = nullptr;  // pinned
+    int cap = 0;
+    int n = 0;
+    size_t lds[KIND_N] = {0, 0, 0};
+    int ntile[KIND_N] = {0, 0, 0};
+    std::vector<EngJob*> jobs;
+    BatchDev* bd() const { return (BatchDev*)dev; }
+    JobDev* jd() const { return (JobDev*)(dev + 64); }
+    static size_t bytes(int cap) { return 64 + (size_t)cap * sizeof(JobDev); }
+};
+
+}  // namespace
+
+struct dwhip_engine {
+    dwhip_engine_config cfg;
+    hipStream_t s = nullptr;
+    bool own_stream = false;
+    char* arena_mem = nullptr;
+    Arena arena;
+    std::deque<EngJob*> queue;
+    std::vector<EngJob*> active;
+    std::map<int64_t, EngJob*> done;
+    Table tsetup, tweights, tae, tfh;
+    long long rounds = 0;
+};
+
+namespace {
+
+int table_alloc(Table* t, int cap)
+{
+    t->cap = cap;
+    ECHK(hipMalloc((void**)&t->dev, Table::bytes(cap)));
+    ECHK(hipHostMalloc((void**)&t->host, Table::bytes(cap), hipHostMallocDefault));
+    return DWHIP_OK;
+}
+
+void table_free(Table* t)
+{
+    if (t->dev) (void)hipFree(t->dev);
+    if (t->host) (void)hipHostFree(t->host);
+    t->dev = t->host = nullptr;
+}
+
+// Fills t with jobs and queues its upload on s.
+int table_build(Table* t, const std::vector<EngJob*>& jobs, hipStream_t s)
+{
+    if ((int)jobs.size() > t->cap) return DWHIP_ERR_ARG;
+    t->jobs = jobs;
+    t->n = (int)jobs.size();
+    BatchDev* bh = (BatchDev*)t->host;
+    JobDev* jh = (JobDev*)(t->host + 64);
+    memset(bh, 0, sizeof(*bh));
+    bh->njobs = t->n;
+    for (int k = 0; k < KIND_N; ++k) {
+        t->ntile[k] = 0;
+        t->lds[k] = 0;
+    }
+    for (int i = 0; i < t->n; ++i) {
+        jh[i] = jobs[i]->j.d;
+        for (int k = 0; k < KIND_N; ++k) {
+            jh[i].tile0[k] = t->ntile[k];
+            t->ntile[k] += jh[i].ntile[k];
+            t->lds[k] = std::max(t->lds[k], jobs[i]->j.tiles.lds[k]);
+        }
+    }
+    for (int k = 0; k < KIND_N; ++k) bh->ntile[k] = t->ntile[k];
+    if (t->n == 0) return DWHIP_OK;
+    ECHK(hipMemcpyAsync(t->dev, t->host, Table::bytes(t->n), hipMemcpyHostToDevice, s));
+    return DWHIP_OK;
+}
+
+// One pass over a table, with the profiling hooks of the per-image path.
+int table_pass(const Table& t, bool xkind, int mode, hipStream_t s)
+{
+    const int kind = xkind ? KIND_X : (mode == CM_YFWD || mode == CM_YINV ? KIND_Y : KIND_Z);
+    if (t.n == 0 || t.ntile[kind] == 0) return DWHIP_OK;
+    const int slot = dwhip_prof_begin(s);
+    hipError_t e = xkind ? dwhip_launch_xpass(mode, t.bd(), t.jd(), t.ntile[kind], t.lds[kind], s)
+                         : dwhip_launch_colpass(mode, t.bd(), t.jd(), t.ntile[kind], t.lds[kind], s);
+    if (slot >= 0) {
+        double by = 0;
+        for (EngJob* j : t.jobs) by += dwhip_pass_bytes(xkind ? 1 : 0, mode, j->j.d, j->j.host_iter);
+        dwhip_prof_end(s, slot, (xkind ? 0 : 7) + mode, by);
+    }
+    return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, xkind ? "batched x pass" : "batched column pass", __LINE__);
+}
+
+int iter_target(const EngJob* j)
+{
+    return j->prm.iter_type == DWHIP_ITER_FIXED ? j->prm.niter : j->prm.maxiter;
+}
+
+// dw_iterator_next (src/dw.c:235-268) before iteration `iter`
+bool keep_going(const EngJob* j, int iter)
+{
+    const dwhip_params* p = &j->prm;
+    if (p->iter_type == DWHIP_ITER_REL) {
+        float last = j->prev_err;
+        if (iter < 2) last = 2 * j->last_err * p->err_rel;
+        if (fabs(j->last_err - last) / j->last_err < p->err_rel) return false;
+    } else if (p->iter_type == DWHIP_ITER_ABS) {
+        if (iter > 0 && j->last_err < p->err_abs) return false;
+    }
+    return iter < iter_target(j);
+}
+
+int admit(dwhip_engine* E, std::vector<EngJob*>& fresh)
+{
+    while (!E->queue.empty() && (int)E->active.size() < E->cfg.max_active) {
+        EngJob* j = E->queue.front();
+        size_t off = 0;
+        if (!E->arena.alloc(j->j.bytes, &off)) break;
+        E->queue.pop_front();
+        j->off = off;
+        RCHK(dwhip_job_bind(&j->j, E->arena_mem + off, E->s));
+        E->active.push_back(j);
+        fresh.push_back(j);
+    }
+    return DWHIP_OK;
+}
+
+// Set-up of newly admitted jobs (method_shb.c:108-294), batched by table.
+int setup(dwhip_engine* E, const std::vector<EngJob*>& fresh)
+{
+    if (fresh.empty()) return DWHIP_OK;
+    hipStream_t s = E->s;
+    RCHK(table_build(&E->tsetup, fresh, s));
+    for (int i = 0; i < (int)fresh.size(); ++i) {
+        ECHK(dwhip_launch_stats(E->tsetup.jd(), i, DWHIP_STAT_BLOCKS, s));
+        ECHK(dwhip_launch_start(E->tsetup.jd(), i, 1024, s));
+    }
+    RCHK(table_pass(E->tsetup, true, XM_PSF, s));
+    RCHK(table_pass(E->tsetup, false, CM_YFWD, s));
+    RCHK(table_pass(E->tsetup, false, CM_ZOTF, s));
+    std::vector<EngJob*> wj;
+    for (EngJob* j : fresh)
+        if (j->j.d.has_w) wj.push_back(j);
+    if (!wj.empty()) {
+        RCHK(table_build(&E->tweights, wj, s));
+        RCHK(table_pass(E->tweights, true, XM_ONES, s));
+        RCHK(table_pass(E->tweights, false, CM_YFWD, s));
+        RCHK(table_pass(E->tweights, false, CM_ZCONVJ, s));
+        RCHK(table_pass(E->tweights, false, CM_YINV, s));
+        RCHK(table_pass(E->tweights, true, XM_WEIGHTS, s));
+    }
+    return DWHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void dwhip_engine_config_default(dwhip_engine_config* c)
+{
+    memset(c, 0, sizeof(*c));
+    c->hbm_budget = (size_t)32 << 30;
+    c->max_active = 64;
+}
+
+int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engine** out)
+{
+    if (!cfg || !out || cfg->hbm_budget < ((size_t)1 << 20) || cfg->max_active < 1 || cfg->max_active > 4096)
+        return DWHIP_ERR_ARG;
+    int rc = dwhip_init();
+    if (rc != DWHIP_OK) return rc;
+    dwhip_engine* E = new dwhip_engine();
+    E->cfg = *cfg;
+    if (stream) {
+        E->s = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreate(&E->s) != hipSuccess) {
+            delete E;
+            return DWHIP_ERR_HIP;
+        }
+        E->own_stream = true;
+    }
+    hipError_t e = hipMalloc((void**)&E->arena_mem, cfg->hbm_budget);
+    if (e != hipSuccess) {
+        if (E->own_stream) (void)hipStreamDestroy(E->s);
+        delete E;
+        return e == hipErrorOutOfMemory ? DWHIP_ERR_NOMEM : DWHIP_ERR_HIP;
+    }
+    E->arena.init(cfg->hbm_budget);
+    const int cap = cfg->max_active;
+    if (table_alloc(&E->tsetup, cap) || table_alloc(&E->tweights, cap) || table_alloc(&E->tae, cap) ||
+        table_alloc(&E->tfh, cap)) {
+        dwhip_engine_destroy(E);
+        return DWHIP_ERR_NOMEM;
+    }
+    *out = E;
+    return DWHIP_OK;
+}
+
+void dwhip_engine_destroy(dwhip_engine* E)
+{
+    if (!E) return;
+    (void)hipStreamSynchronize(E->s);
+    for (EngJob* j : E->queue) delete j;
+    for (EngJob* j : E->active) delete j;
+    for (auto& kv : E->done) delete kv.second;
+    table_free(&E->tsetup);
+    table_free(&E->tweights);
+    table_free(&E->tae);
+    table_free(&E->tfh);
+    if (E->arena_mem) (void)hipFree(E->arena_mem);
+    if (E->own_stream) (void)hipStreamDestroy(E->s);
+    delete E;
+}
+
+int dwhip_engine_submit(dwhip_engine* E, const float* im, int64_t M, int64_t N, int64_t P, const float* psf,
+                        int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm, float* out, int64_t tag)
+{
+    if (!E || !im || !psf || !prm || !out) return DWHIP_ERR_ARG;
+    if (M < 1 || N < 1 || P < 1 || pM < 1 || pN < 1 || pP < 1) return DWHIP_ERR_ARG;
+    if (E->done.count(tag)) return DWHIP_ERR_ARG;
+    for (EngJob* j : E->queue)
+        if (j->tag == tag) return DWHIP_ERR_ARG;
+    for (EngJob* j : E->active)
+        if (j->tag == tag) return DWHIP_ERR_ARG;
+    const bool fixed = prm->iter_type == DWHIP_ITER_FIXED;
+    if ((fixed && prm->niter < 1) || (!fixed && prm->maxiter < 1)) return DWHIP_ERR_ARG;
+    EngJob* j = new EngJob();
+    j->prm = *prm;
+    j->tag = tag;
+    j->out = out;
+    const float* start = prm->start != DWHIP_START_FLAT ? prm->start_image : nullptr;
+    int rc = dwhip_job_plan(&j->j, im, M, N, P, psf, pM, pN, pP, prm, start);
+    if (rc == DWHIP_OK && align_up(j->j.bytes, 4096) > E->cfg.hbm_budget) rc = DWHIP_ERR_NOMEM;
+    if (rc != DWHIP_OK) {
+        delete j;
+        return rc;
+    }
+    E->queue.push_back(j);
+    return DWHIP_OK;
+}
+
+int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_done)
+{
+    if (!E) return DWHIP_ERR_ARG;
+    if (n_done) *n_done = 0;
+    hipStream_t s = E->s;
+    std::vector<EngJob*> fresh;
+    RCHK(admit(E, fresh));
+    RCHK(setup(E, fresh));
+    if (E->active.empty()) {
+        ECHK(hipStreamSynchronize(s));
+        return DWHIP_OK;
+    }
+    // one iteration of every active job; the adjoint half (F, G, H) only
+    // for jobs with a further iteration to feed
+    std::vector<EngJob*> fh;
+    for (EngJob* j : E->active) {
+        const bool fixed = j->prm.iter_type == DWHIP_ITER_FIXED;
+        if (!fixed || j->j.host_iter + 1 < j->prm.niter) fh.push_back(j);
+    }
+    RCHK(table_build(&E->tae, E->active, s));
+    RCHK(table_build(&E->tfh, fh, s));
+    RCHK(table_pass(E->tae, true, XM_UPDATE, s));
+    RCHK(table_pass(E->tae, false, CM_YFWD, s));
+    RCHK(table_pass(E->tae, false, CM_ZCONV, s));
+    RCHK(table_pass(E->tae, false, CM_YINV, s));
+    RCHK(table_pass(E->tae, true, XM_RATIO, s));
+    RCHK(table_pass(E->tfh, false, CM_YFWD, s));
+    RCHK(table_pass(E->tfh, false, CM_ZCONVJ, s));
+    RCHK(table_pass(E->tfh, false, CM_YINV, s));
+    ECHK(dwhip_launch_round_end(E->tae.bd(), E->tae.jd(), E->tae.n, 1, s));
+    for (EngJob* j : E->active) {
+        j->j.host_iter += 1;
+        j->nrun = j->j.host_iter;
+    }
+    E->rounds += 1;
+    // errors of this round, for REL/ABS jobs
+    std::vector<float> errs(E->active.size(), 0.f);
+    bool need_err = false;
+    for (size_t i = 0; i < E->active.size(); ++i) {
+        EngJob* j = E->active[i];
+        if (j->prm.iter_type != DWHIP_ITER_FIXED) {
+            ECHK(hipMemcpyAsync(&errs[i], j->j.d.err + (j->nrun - 1), sizeof(float), hipMemcpyDeviceToHost, s));
+            need_err = true;
+        }
+    }
+    if (need_err) ECHK(hipStreamSynchronize(s));
+    // finished jobs: crop x_{n-1} into out
+    std::vector<EngJob*> finished, still;
+    for (size_t i = 0; i < E->active.size(); ++i) {
+        EngJob* j = E->active[i];
+        bool fin;
+        if (j->prm.iter_type == DWHIP_ITER_FIXED) {
+            fin = j->nrun >= j->prm.niter;
+        } else {
+            j->prev_err = j->last_err;
+            j->last_err = errs[i];
+            fin = !keep_going(j, j->nrun);
+        }
+        if (fin) {
+            ECHK(dwhip_launch_crop(j->j.jd, 0, j->nrun - 1, j->out, 1024, s));
+            finished.push_back(j);
+        } else {
+            still.push_back(j);
+        }
+    }
+    ECHK(hipStreamSynchronize(s));
+    if (dwhip_prof_on()) dwhip_prof_collect();
+    for (EngJob* j : finished) {
+        j->err.resize(j->nrun);
+        ECHK(hipMemcpy(j->err.data(), j->j.d.err, j->nrun * sizeof(float), hipMemcpyDeviceToHost));
+        JobState hs;
+        ECHK(hipMemcpy(&hs, j->j.d.st, sizeof(hs), hipMemcpyDeviceToHost));
+        j->bg = hs.bg;
+        E->arena.release(j->off, j->j.bytes);
+        j->j.mem = nullptr;
+        E->done[j->tag] = j;
+        if (done_tags && n_done && *n_done < max_done) done_tags[(*n_done)++] = j->tag;
+    }
+    E->active = still;
+    return DWHIP_OK;
+}
+
+int dwhip_engine_status(const dwhip_engine* E, int* queued, int* active, size_t* bytes_in_use)
+{
+    if (!E) return DWHIP_ERR_ARG;
+    if (queued) *queued = (int)E->queue.size();
+    if (active) *active = (int)E->active.size();
+    if (bytes_in_use) *bytes_in_use = E->arena.used;
+    return DWHIP_OK;
+}
+
+int dwhip_engine_result(dwhip_engine* E, int64_t tag, dwhip_stats* st, int release)
+{
+    if (!E) return DWHIP_ERR_ARG;
+    auto it = E->done.find(tag);
+    if (it == E->done.end()) return DWHIP_ERR_STATE;
+    EngJob* j = it->second;
+    if (st) {
+        st->bg = j->bg;
+        st->iterations = j->nrun;
+        for (int d = 0; d < 3; ++d) {
+            st->work[d] = j->j.work[d];
+            st->ref_work[d] = j->j.ref_work[d];
+        }
+        st->setup_ms = 0;
+        st->iter_ms = 0;
+        if (st->err_trace && st->err_trace_len > 0) {
+            const int n = std::min((int)j->err.size(), st->err_trace_len);
+            memcpy(st->err_trace, j->err.data(), n * sizeof(float));
+        }
+    }
+    if (release) {
+        delete j;
+        E->done.erase(it);
+    }
+    return DWHIP_OK;
+}
+
+}  // extern "C"

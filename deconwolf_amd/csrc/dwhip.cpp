@@ -290,12 +290,16 @@ void dwhip_job_free(DwhipJob* j)
     j->mem = nullptr;
 }
 
-// Builds the job (device buffers, plans) for device-resident im/psf.
-int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P,
-                     const float* psf_dev, int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm,
-                     const float* start_dev)
+// Plans a job for device-resident im/psf: sizes, tiles, FFT plans and
+// twiddles (into j->tw), and the byte layout of its device block
+// (j->bytes).  Allocates nothing.
+int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P, const float* psf_dev,
+                   int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm, const float* start_dev)
 {
-    memset(j, 0, sizeof(*j));
+    j->mem = nullptr;
+    j->host_iter = 0;
+    memset(&j->d, 0, sizeof(j->d));
+    j->tw.clear();
     int64_t ref[3], w[3];
     int rc = dwhip_work_size(M, N, P, pM, pN, pP, prm->border_quality, w, ref);
     if (rc == DWHIP_ERR_ARG) return rc;
@@ -345,70 +349,90 @@ int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int
     d.psf = psf_dev;
     d.start_img = start_dev;
 
-    // twiddles
-    std::vector<float2> tw;
+    std::vector<float2>& tw = j->tw;
     if (make_plan_pair(d.h, true, &d.pxf, &d.pxi, tw) || make_plan_pair(d.ny, false, &d.pyf, &d.pyi, tw) ||
         make_plan_pair(d.nz, false, &d.pzf, &d.pzi, tw))
         return DWHIP_ERR_SIZE;
-    const size_t rtw_off = tw.size();
+    j->rtw_off = tw.size();
     for (int k = 0; k < d.h; ++k) {
         const double a = -2.0 * M_PI * (double)k / (double)d.nx;
         tw.push_back(make_float2((float)cos(a), (float)sin(a)));
     }
 
-    // one allocation, carved into aligned pieces
+    // layout of the job's block
     const size_t V = (size_t)d.nx * d.ny * d.nz;
     const size_t Vc = (size_t)d.hcs * d.ny * d.nz;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off = round_up(off + bytes, 256); return o; };
-    const size_t oC = take(Vc * sizeof(float2));
-    const size_t oK = take(Vc * sizeof(float2));
-    const size_t oU0 = take(V * sizeof(float));
-    const size_t oU1 = take(V * sizeof(float));
-    const size_t oW = d.has_w ? take(V * sizeof(float)) : 0;
-    const size_t oPart = take((size_t)d.ntile[KIND_X] * sizeof(double));
-    const size_t oStat = take(DWHIP_STAT_BLOCKS * 4 * sizeof(double));
-    const size_t oErr = take((size_t)niter * sizeof(float));
-    const size_t oErrd = take((size_t)niter * sizeof(double));
-    const size_t oTw = take(tw.size() * sizeof(float2));
-    const size_t oSt = take(sizeof(JobState));
-    const size_t oTab = take(sizeof(BatchDev) + 64 + sizeof(JobDev));
+    DwhipLayout& L = j->lay;
+    L.C = take(Vc * sizeof(float2));
+    L.K = take(Vc * sizeof(float2));
+    L.U0 = take(V * sizeof(float));
+    L.U1 = take(V * sizeof(float));
+    L.W = d.has_w ? take(V * sizeof(float)) : 0;
+    L.part = take((size_t)d.ntile[KIND_X] * sizeof(double));
+    L.stat = take(DWHIP_STAT_BLOCKS * 4 * sizeof(double));
+    L.err = take((size_t)niter * sizeof(float));
+    L.errd = take((size_t)niter * sizeof(double));
+    L.tw = take(tw.size() * sizeof(float2));
+    L.st = take(sizeof(JobState));
+    L.tab = take(sizeof(BatchDev) + 64 + sizeof(JobDev));
     j->bytes = off;
-    hipError_t e = hipMalloc(&j->mem, off);
-    if (e != hipSuccess) {
-        j->mem = nullptr;
-        return e == hipErrorOutOfMemory ? DWHIP_ERR_NOMEM : DWHIP_ERR_HIP;
-    }
-    char* base = (char*)j->mem;
-    d.C = (float2*)(base + oC);
-    d.K = (float2*)(base + oK);
-    d.U0 = (float*)(base + oU0);
-    d.U1 = (float*)(base + oU1);
-    d.W = d.has_w ? (float*)(base + oW) : nullptr;
-    d.part = (double*)(base + oPart);
-    d.stat = (double*)(base + oStat);
-    d.err = (float*)(base + oErr);
-    d.errd = (double*)(base + oErrd);
-    d.tw = (const float2*)(base + oTw);
-    d.rtw = d.tw + rtw_off;
-    d.st = (JobState*)(base + oSt);
-    j->bd = (BatchDev*)(base + oTab);
-    j->jd = (JobDev*)(base + oTab + 64);
+    return DWHIP_OK;
+}
 
-    HIPCHK(hipMemcpy(base + oTw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(base + oC, 0, 2 * Vc * sizeof(float2)));
-    HIPCHK(hipMemset(d.part, 0, (size_t)d.ntile[KIND_X] * sizeof(double)));
-    HIPCHK(hipMemset(d.st, 0, sizeof(JobState)));
-    BatchDev bh;
-    memset(&bh, 0, sizeof(bh));
-    bh.njobs = 1;
+// Points the job at its device block and queues the uploads (twiddles, the
+// one-job table) on stream s.  The host copies must stay alive until s
+// has run them (they live in the job).
+int dwhip_job_bind(DwhipJob* j, void* mem, hipStream_t s)
+{
+    j->mem = mem;
+    char* base = (char*)mem;
+    const DwhipLayout& L = j->lay;
+    JobDev& d = j->d;
+    d.C = (float2*)(base + L.C);
+    d.K = (float2*)(base + L.K);
+    d.U0 = (float*)(base + L.U0);
+    d.U1 = (float*)(base + L.U1);
+    d.W = d.has_w ? (float*)(base + L.W) : nullptr;
+    d.part = (double*)(base + L.part);
+    d.stat = (double*)(base + L.stat);
+    d.err = (float*)(base + L.err);
+    d.errd = (double*)(base + L.errd);
+    d.tw = (const float2*)(base + L.tw);
+    d.rtw = d.tw + j->rtw_off;
+    d.st = (JobState*)(base + L.st);
+    j->bd = (BatchDev*)(base + L.tab);
+    j->jd = (JobDev*)(base + L.tab + 64);
+    memset(&j->bh, 0, sizeof(j->bh));
+    j->bh.njobs = 1;
     for (int k = 0; k < KIND_N; ++k) {
         d.tile0[k] = 0;
-        bh.ntile[k] = d.ntile[k];
+        j->bh.ntile[k] = d.ntile[k];
     }
-    HIPCHK(hipMemcpy(j->bd, &bh, sizeof(bh), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(j->jd, &d, sizeof(d), hipMemcpyHostToDevice));
+    j->dcopy = d;
+    HIPCHK(hipMemcpyAsync(base + L.tw, j->tw.data(), j->tw.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(j->bd, &j->bh, sizeof(j->bh), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(j->jd, &j->dcopy, sizeof(j->dcopy), hipMemcpyHostToDevice, s));
     return DWHIP_OK;
+}
+
+// Plans, allocates and binds a job for the per-image path.
+int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P,
+                     const float* psf_dev, int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm,
+                     const float* start_dev)
+{
+    int rc = dwhip_job_plan(j, im_dev, M, N, P, psf_dev, pM, pN, pP, prm, start_dev);
+    if (rc != DWHIP_OK) return rc;
+    void* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, j->bytes);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? DWHIP_ERR_NOMEM : DWHIP_ERR_HIP;
+    rc = dwhip_job_bind(j, mem, nullptr);
+    if (rc == DWHIP_OK) {
+        e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) rc = dwhip_hip_fail(e, "job upload", __LINE__);
+    }
+    return rc;
 }
 
 // Algorithmic HBM bytes of one pass over job d: each operand read or

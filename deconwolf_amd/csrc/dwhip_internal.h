@@ -18,18 +18,28 @@ struct DwhipTiles {
     size_t lds[KIND_N];
 };
 
-// One job: host copy of its descriptor plus its device allocation, which
-// also holds a one-job table (bd, jd) for the per-image path.
+// Byte offsets of a job's buffers inside its device block.
+struct DwhipLayout {
+    size_t C, K, U0, U1, W, part, stat, err, errd, tw, st, tab;
+};
+
+// One job: host copy of its descriptor plus its device block, which also
+// holds a one-job table (bd, jd) for the per-image path.
 struct DwhipJob {
     JobDev d;
     DwhipTiles tiles;
+    DwhipLayout lay;
     int64_t work[3];
     int64_t ref_work[3];
-    void* mem;
-    size_t bytes;
-    BatchDev* bd;
-    JobDev* jd;
-    int host_iter;   // iteration the next round runs (host copy of st->iter)
+    void* mem = nullptr;
+    size_t bytes = 0;
+    BatchDev* bd = nullptr;
+    JobDev* jd = nullptr;
+    int host_iter = 0;        // iteration the next round runs (host copy of st->iter)
+    std::vector<float2> tw;   // twiddles, uploaded by dwhip_job_bind
+    size_t rtw_off = 0;
+    BatchDev bh;              // host copies of the one-job table being uploaded
+    JobDev dcopy;
 };
 
 int dwhip_hip_fail(hipError_t e, const char* what, int line);
@@ -45,8 +55,15 @@ double dwhip_pass_bytes(int xkind, int mode, const JobDev& d, int iter);
 int dwhip_make_plan(int n, const std::vector<int>& radices, FftPlanDev* p, std::vector<float2>& tw);
 std::vector<int> dwhip_radices(int n, bool rows);
 void dwhip_tile_config(const int64_t* w, DwhipTiles* t);
+int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P, const float* psf_dev,
+                   int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm, const float* start_dev);
+int dwhip_job_bind(DwhipJob* j, void* mem, hipStream_t s);
 int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P, const float* psf_dev,
                      int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm, const float* start_dev);
+int dwhip_pass_x(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* jd, int ntile, size_t lds,
+                 hipStream_t s);
+int dwhip_pass_c(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* jd, int ntile, size_t lds,
+                 hipStream_t s);
 void dwhip_job_free(DwhipJob* j);
 int dwhip_job_setup(const DwhipJob* j, hipStream_t s);
 int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s);

@@ -144,6 +144,48 @@ int dwhip_fft3_device(float *vol, float *spec, int64_t nx, int64_t ny, int64_t n
                       void *stream);
 
 /*
+ * Batched and served execution.  No counterpart in deconwolf, which runs
+ * one job (or one tile, src/dw.c:2093-2153) at a time; this engine is for
+ * callers with many jobs: tiles of one image, fields and channels of an
+ * acquisition, or a request stream.
+ *
+ * An engine owns `hbm_budget` bytes of device memory.  Submitted jobs
+ * (device pointers; the caller keeps im, psf, start image and out alive
+ * until the job is reported done) wait in a FIFO queue and are admitted
+ * while their state fits in the budget and fewer than `max_active` run.
+ * Every dwhip_engine_step() admits what fits, sets the new jobs up, runs
+ * one SHB iteration of every active job (one launch per pass for all of
+ * them), and completes the jobs that reached their iteration count or
+ * stopping rule: their result is in `out` when the step returns and their
+ * memory is free for the next admissions.  Results equal those of
+ * dwhip_shb_run_device for the same job, whatever else runs beside it.
+ * Not thread-safe: one host thread drives an engine.
+ */
+typedef struct dwhip_engine dwhip_engine;
+
+typedef struct {
+    size_t hbm_budget;  /* device bytes for job state (allocated at creation) */
+    int max_active;     /* jobs iterating together */
+    int reserved[6];
+} dwhip_engine_config;
+
+void dwhip_engine_config_default(dwhip_engine_config *cfg);
+int dwhip_engine_create(const dwhip_engine_config *cfg, void *stream, dwhip_engine **engine);
+void dwhip_engine_destroy(dwhip_engine *engine);
+/* Queues a job; `tag` identifies it in results.  DWHIP_ERR_NOMEM when the
+ * job alone exceeds the budget, DWHIP_ERR_SIZE / DWHIP_ERR_ARG as for
+ * dwhip_shb_run. */
+int dwhip_engine_submit(dwhip_engine *engine, const float *im, int64_t M, int64_t N, int64_t P,
+                        const float *psf, int64_t pM, int64_t pN, int64_t pP,
+                        const dwhip_params *params, float *out, int64_t tag);
+/* One round; writes up to max_done tags of jobs completed in this round. */
+int dwhip_engine_step(dwhip_engine *engine, int64_t *done_tags, int max_done, int *n_done);
+int dwhip_engine_status(const dwhip_engine *engine, int *queued, int *active, size_t *bytes_in_use);
+/* Stats of a completed job (bg, iterations, sizes, error trace); release
+ * != 0 forgets the job afterwards.  DWHIP_ERR_STATE if not completed. */
+int dwhip_engine_result(dwhip_engine *engine, int64_t tag, dwhip_stats *stats, int release);
+
+/*
  * Kernel timing for measurement.  While enabled, every pass launch is
  * bracketed by HIP events on the stream it runs on and its algorithmic
  * bytes (each operand read or written once) are added up per kernel.

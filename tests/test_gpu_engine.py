@@ -1,0 +1,143 @@
+"""GPU tests of the batched/served engine (dwhip_engine_*): every job's
+result in a mixed batch is bit-identical to the same job run alone through
+dwhip_shb_run_device, whatever runs beside it, however it is admitted."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import deconwolf_amd as dw
+from deconwolf_amd import synth
+from deconwolf_amd.serve import Engine, serve
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def make_job(seed, shape, psfshape, niter, **opt):
+    rng = np.random.default_rng(seed)
+    P, N, M = shape
+    pP, pN, pM = psfshape
+    k = synth.gaussian_psf(pM, pN, pP, 1.2 + 0.1 * seed, 1.5, 1.8)
+    V = synth.point_sources(shape, 25, rng=rng)
+    im = synth.noisy_image(synth.blur(V, k), 40.0 + seed, rng=rng) + np.float32(5.0)
+    s = dw.DwOpts(nIter=niter, iter_type=dw.ITER_FIXED)
+    for a, v in opt.items():
+        setattr(s, a, v)
+    return torch.from_numpy(im).to(DEV), torch.from_numpy(k).to(DEV), s
+
+
+JOBS = [
+    (1, (16, 36, 40), (7, 9, 9), 12, {}),
+    (2, (10, 24, 30), (5, 7, 7), 5, {"metric": dw.METRIC_MSE}),
+    (3, (12, 40, 28), (7, 7, 5), 20, {"positivity": 0, "alphamax": 0.7}),
+    (4, (8, 20, 24), (5, 5, 5), 1, {}),
+    (5, (11, 26, 43), (7, 7, 9), 7, {"borderQuality": 1}),
+    (6, (12, 27, 40), (5, 5, 5), 6, {"borderQuality": 0, "start_condition": dw.START_IDENTITY}),
+]
+
+
+def solo(im, k, s):
+    import copy
+    s2 = copy.copy(s)
+    out = dw.deconvolve_shb_hip_device(im, k, s2)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), s2
+
+
+def test_batch_equals_alone():
+    print("engine: mixed batch of", len(JOBS), flush=True)
+    jobs = [make_job(sd, sh, ps, n, **o) for sd, sh, ps, n, o in JOBS]
+    eng = Engine(hbm_budget_gb=2.0, max_active=16)
+    for t, (im, k, s) in enumerate(jobs):
+        eng.submit(t, im, k, s)
+    done = []
+    rounds = 0
+    while len(done) < len(jobs):
+        done += eng.step()
+        rounds += 1
+        assert rounds < 100
+    assert rounds == max(j[3] for j in JOBS)   # all admitted at once, one iteration per round
+    for t, (im, k, s) in enumerate(jobs):
+        ref, s2 = solo(im, k, s)
+        out, st = eng.result(t)
+        assert st["iterations"] == s2.stats["iterations"]
+        assert np.array_equal(out.cpu().numpy(), ref), t
+        np.testing.assert_array_equal(st["err_trace"], s2.err_trace)
+        assert st["bg"] == s2.bg
+    eng.close()
+
+
+def test_budget_limited_admission_and_staggered_submission():
+    print("engine: budget-limited admission", flush=True)
+    jobs = [make_job(10 + i, (16, 36, 40), (7, 9, 9), 3 + i) for i in range(5)]
+    w, r, _ = dw.work_size((16, 36, 40), (7, 9, 9))
+    V = w[0] * w[1] * w[2]
+    # room for about two of these jobs (a job is ~5 V floats)
+    eng = Engine(hbm_budget_gb=2.5 * 5 * V * 4 / (1 << 30), max_active=4)
+    for t in range(2):
+        eng.submit(t, *jobs[t])
+    seen_active = 0
+    seen_queued = 0
+    done = []
+    t_next = 2
+    for _ in range(200):
+        done += eng.step()
+        q, a, b = eng.status()
+        seen_active = max(seen_active, a)
+        seen_queued = max(seen_queued, q)
+        if t_next < len(jobs):
+            eng.submit(t_next, *jobs[t_next])
+            t_next += 1
+        if len(done) == len(jobs):
+            break
+    assert len(done) == len(jobs)
+    assert seen_active <= 2 and seen_queued >= 1   # the budget, not max_active, limited admission
+    for t, (im, k, s) in enumerate(jobs):
+        ref, _ = solo(im, k, s)
+        out, _ = eng.result(t)
+        assert np.array_equal(out.cpu().numpy(), ref), t
+    eng.close()
+
+
+def test_rel_stopping_in_engine():
+    im, k, s = make_job(21, (10, 24, 30), (5, 7, 7), 1)
+    s.iter_type = dw.ITER_REL
+    s.err_rel = 0.05
+    s.maxiter = 40
+    ref, s2 = solo(im, k, s)
+    eng = Engine(hbm_budget_gb=0.5, max_active=2)
+    eng.submit(0, im, k, s)
+    im2, k2, s3 = make_job(22, (16, 36, 40), (7, 9, 9), 30)
+    eng.submit(1, im2, k2, s3)
+    done = []
+    while len(done) < 2:
+        done += eng.step()
+    out, st = eng.result(0)
+    assert st["iterations"] == s2.stats["iterations"]
+    assert np.array_equal(out.cpu().numpy(), ref)
+    eng.close()
+
+
+def test_oversized_job_refused():
+    im, k, s = make_job(30, (32, 90, 100), (9, 9, 9), 2)   # about 9 MB of job state
+    eng = Engine(hbm_budget_gb=0.002, max_active=2)
+    with pytest.raises(dw.DwhipError) as e:
+        eng.submit(0, im, k, s)
+    assert "NOMEM" in str(e.value)
+    eng.close()
+
+
+def test_serve_loop_latencies():
+    jobs = [make_job(40 + i, (12, 30, 34), (5, 7, 7), 4 + (i % 3)) for i in range(8)]
+    eng = Engine(hbm_budget_gb=0.5, max_active=8)
+    reqs = [(i, im, k, s) for i, (im, k, s) in enumerate(jobs)]
+    arrivals = [0.002 * i for i in range(len(jobs))]
+    times = serve(eng, reqs, arrivals)
+    assert len(times) == len(jobs)
+    for i, (arr, dn) in times.items():
+        assert dn is not None and dn >= arr
+        ref, _ = solo(*jobs[i])
+        out, _ = eng.result(i)
+        assert np.array_equal(out.cpu().numpy(), ref)
+    eng.close()
