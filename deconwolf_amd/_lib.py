@@ -57,6 +57,10 @@ class EngineConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
 
 
+class DwhipTile(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_int64 * 6), ("xpos", ctypes.c_int64 * 6), ("xsize", ctypes.c_int64 * 3)]
+
+
 class DwhipError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -90,12 +94,16 @@ SIGNATURES = [
     ("dwhip_engine_step", _int, [_vp, _P64, _int, ctypes.POINTER(_int)]),
     ("dwhip_engine_status", _int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_size_t)]),
     ("dwhip_engine_result", _int, [_vp, _i64, ctypes.POINTER(DwhipStats), _int]),
+    ("dwhip_tiling_create", _int, [_i64, _i64, _i64, _i64, _i64, ctypes.POINTER(DwhipTile), _int, ctypes.POINTER(_int)]),
+    ("dwhip_tile_extract_device", _int, [_vp, _i64, _i64, _i64, ctypes.POINTER(DwhipTile), ctypes.c_float, _vp, _vp]),
+    ("dwhip_tiles_blend_device", _int, [ctypes.POINTER(DwhipTile), ctypes.POINTER(_vp), _int, _i64, _i64, _i64,
+                                        ctypes.c_float, _vp, _vp]),
     ("dwhip_profile_enable", _int, [_int]),
     ("dwhip_profile_get", _int, [_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_longlong),
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 ]
 
-N_PROFILE_KERNELS = 14
+N_PROFILE_KERNELS = 16
 
 
 def profile_enable(on=True):

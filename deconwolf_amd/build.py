@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdwhip.so")
 
 N_XMODES = 7
-N_CMODES = 7
+N_CMODES = 9
 
 # -packed-fp32-ops: no packed FP32 instructions in device code.  The host
 # pass of hipcc warns that it does not know the feature; that is harmless.
@@ -44,7 +44,7 @@ def units():
     for m in range(N_CMODES):
         u.append(("k_colpass.hip", ["-DCMODE=%d" % m], "k_colpass_%d.o" % m))
     u.append(("k_misc.hip", [], "k_misc.o"))
-    for src in ("dwhip.cpp", "profile.cpp", "batch.cpp"):
+    for src in ("dwhip.cpp", "profile.cpp", "batch.cpp", "tiling.cpp"):
         if os.path.exists(os.path.join(CSRC, src)):
             u.append((src, [], src.replace(".cpp", ".o")))
     return u

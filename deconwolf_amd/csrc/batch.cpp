@@ -174,7 +174,7 @@ int table_build(Table* t, const std::vector<EngJob*>& jobs, hipStream_t s)
 // One pass over a table, with the profiling hooks of the per-image path.
 int table_pass(const Table& t, bool xkind, int mode, hipStream_t s)
 {
-    const int kind = xkind ? KIND_X : (mode == CM_YFWD || mode == CM_YINV ? KIND_Y : KIND_Z);
+    const int kind = xkind ? KIND_X : (cmode_is_y(mode) ? KIND_Y : KIND_Z);
     if (t.n == 0 || t.ntile[kind] == 0) return DWHIP_OK;
     const int slot = dwhip_prof_begin(s);
     hipError_t e = xkind ? dwhip_launch_xpass(mode, t.bd(), t.jd(), t.ntile[kind], t.lds[kind], s)
@@ -360,9 +360,9 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
     RCHK(table_pass(E->tae, true, XM_UPDATE, s));
     RCHK(table_pass(E->tae, false, CM_YFWD, s));
     RCHK(table_pass(E->tae, false, CM_ZCONV, s));
-    RCHK(table_pass(E->tae, false, CM_YINV, s));
+    RCHK(table_pass(E->tae, false, CM_YINV_IMG, s));
     RCHK(table_pass(E->tae, true, XM_RATIO, s));
-    RCHK(table_pass(E->tfh, false, CM_YFWD, s));
+    RCHK(table_pass(E->tfh, false, CM_YFWD_IMG, s));
     RCHK(table_pass(E->tfh, false, CM_ZCONVJ, s));
     RCHK(table_pass(E->tfh, false, CM_YINV, s));
     ECHK(dwhip_launch_round_end(E->tae.bd(), E->tae.jd(), E->tae.n, 1, s));

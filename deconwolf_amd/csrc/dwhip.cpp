@@ -127,7 +127,7 @@ static bool size_ok(const int64_t* w)
     // kernel limits: at most DWHIP_XMAXE*NTHR complex per x-row tile,
     // DWHIP_CMAXE*NTHR per column tile, 2^20 for the index arithmetic
     if (w[0] / 2 > DWHIP_XMAXE * DWHIP_XTHR) return false;
-    if (w[1] > DWHIP_CMAXE * DWHIP_CTHR || w[2] > DWHIP_CMAXE * DWHIP_CTHR) return false;
+    if (w[1] > DWHIP_CMAXE * DWHIP_CTHR || w[2] > DWHIP_CMAXE * DWHIP_ZTHR) return false;
     if (w[0] < 2 || w[1] < 1 || w[2] < 1) return false;
     if (w[0] / 2 > 1 && dwhip_radices((int)(w[0] / 2), true).empty()) return false;
     if (w[1] > 1 && dwhip_radices((int)w[1], false).empty()) return false;
@@ -246,7 +246,7 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
     const int hc = h + 1;
     t->xrows = std::max(1, std::min(64, DWHIP_XMAXE * DWHIP_XTHR / h));
     t->ycols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_CTHR / ny)));
-    t->zcols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_CTHR / nz)));
+    t->zcols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_ZTHR / nz)));
     t->ntile[KIND_X] = (ny * nz + t->xrows - 1) / t->xrows;
     t->ntile[KIND_Y] = ((hc + t->ycols - 1) / t->ycols) * nz;
     t->ntile[KIND_Z] = ((hc + t->zcols - 1) / t->zcols) * ny;
@@ -441,16 +441,27 @@ int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int
 // matters for the update pass (iteration 0 has no inverse transform).
 double dwhip_pass_bytes(int xkind, int mode, const JobDev& d, int iter)
 {
-    const double Vc = (double)d.hc * d.ny * d.nz * sizeof(float2);
+    const double row = (double)d.hc * sizeof(float2);          // one complex x-row
+    const double Vc = row * d.ny * d.nz;
     const double V = (double)d.nx * d.ny * d.nz * sizeof(float);
     const double img = (double)d.M * d.N * d.P * sizeof(float);
-    if (!xkind) return (mode == CM_ZCONV || mode == CM_ZCONVJ) ? 3 * Vc : 2 * Vc;
+    const double slabP = row * d.ny * d.P;                     // z < P
+    const double imgrows = row * d.N * d.P;                    // y < N, z < P
+    if (!xkind) {
+        switch (mode) {
+        case CM_ZCONV: return Vc + Vc + slabP;                 // read C, K; write z < P
+        case CM_ZCONVJ: return slabP + Vc + Vc;                // read z < P, K; write C
+        case CM_YINV_IMG: return slabP + imgrows;              // slabs z < P, write y < N
+        case CM_YFWD_IMG: return imgrows + slabP;
+        default: return 2 * Vc;
+        }
+    }
     switch (mode) {
     case XM_PSF: return Vc + (double)d.pM * d.pN * d.pP * sizeof(float);
     case XM_ONES: return Vc;
     case XM_WEIGHTS: return Vc + V;
     case XM_UPDATE: return iter == 0 ? 2 * V + Vc : 2 * Vc + (d.has_w ? 4 : 3) * V;
-    case XM_RATIO: return 2 * Vc + img;
+    case XM_RATIO: return 2 * imgrows + img;
     case XM_LOAD: return Vc + V;
     case XM_STORE: return Vc + V;
     default: return 0;
@@ -518,10 +529,10 @@ int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s)
     RCHK(dwhip_pass_x(XM_UPDATE, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
     RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
     RCHK(dwhip_pass_c(CM_ZCONV, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
-    RCHK(dwhip_pass_c(CM_YINV, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+    RCHK(dwhip_pass_c(CM_YINV_IMG, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
     RCHK(dwhip_pass_x(XM_RATIO, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
     if (full) {
-        RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+        RCHK(dwhip_pass_c(CM_YFWD_IMG, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
         RCHK(dwhip_pass_c(CM_ZCONVJ, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
         RCHK(dwhip_pass_c(CM_YINV, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
     }

@@ -45,8 +45,8 @@ struct DwhipJob {
 int dwhip_hip_fail(hipError_t e, const char* what, int line);
 
 // profiling of pass launches (profile.cpp); kernel ids: X modes 0..6, then
-// column modes 7..13
-#define DWHIP_PROF_KERNELS 14
+// column modes 7..15
+#define DWHIP_PROF_KERNELS 16
 bool dwhip_prof_on();
 int dwhip_prof_begin(hipStream_t s);
 void dwhip_prof_end(hipStream_t s, int slot, int kernel, double bytes);

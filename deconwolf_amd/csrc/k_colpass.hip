@@ -15,6 +15,6 @@ extern "C" hipError_t DWHIP_CAT(dwhip_launch_c, CMODE)(const BatchDev* bd, const
         once = true;
         dwhip_allow_big_lds(reinterpret_cast<const void*>(k_colpass<CMODE>));
     }
-    hipLaunchKernelGGL(k_colpass<CMODE>, dim3(grid), dim3(CTHR), lds, s, bd, jobs);
+    hipLaunchKernelGGL(k_colpass<CMODE>, dim3(grid), dim3(col_threads<CMODE>()), lds, s, bd, jobs);
     return hipGetLastError();
 }

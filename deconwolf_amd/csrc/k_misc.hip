@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <algorithm>
+
 #include "shb_dev.h"
 #include "shb_kernels.h"
 
@@ -176,6 +178,8 @@ DECL_LAUNCH(dwhip_launch_c3)
 DECL_LAUNCH(dwhip_launch_c4)
 DECL_LAUNCH(dwhip_launch_c5)
 DECL_LAUNCH(dwhip_launch_c6)
+DECL_LAUNCH(dwhip_launch_c7)
+DECL_LAUNCH(dwhip_launch_c8)
 
 extern "C" {
 
@@ -196,7 +200,7 @@ hipError_t dwhip_launch_xpass(int mode, const BatchDev* bd, const JobDev* jobs, 
 
 hipError_t dwhip_launch_colpass(int mode, const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s)
 {
-    static_assert(CM_YFWD == 0 && CM_ZINV == 6, "column modes 0..6");
+    static_assert(CM_YFWD == 0 && CM_YFWD_IMG == 8 && DWHIP_N_CMODES == 9, "column modes 0..8");
     switch (mode) {
     case 0: return dwhip_launch_c0(bd, jobs, grid, lds, s);
     case 1: return dwhip_launch_c1(bd, jobs, grid, lds, s);
@@ -205,6 +209,8 @@ hipError_t dwhip_launch_colpass(int mode, const BatchDev* bd, const JobDev* jobs
     case 4: return dwhip_launch_c4(bd, jobs, grid, lds, s);
     case 5: return dwhip_launch_c5(bd, jobs, grid, lds, s);
     case 6: return dwhip_launch_c6(bd, jobs, grid, lds, s);
+    case 7: return dwhip_launch_c7(bd, jobs, grid, lds, s);
+    case 8: return dwhip_launch_c8(bd, jobs, grid, lds, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -235,3 +241,92 @@ hipError_t dwhip_launch_crop(const JobDev* jobs, int j, int k, float* out, int g
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// XY tiling (deconwolf's tiling.c / deconvolve_tiles, src/dw.c:2000-2229)
+// ---------------------------------------------------------------------------
+// Tile i: out[(m - x0) + xm*((n - y0) + xn*(p - z0))] = im[m + M*(n + N*p)] + offset
+// over the tile's grown box (tiling_get_tile, tiling.c:300-330, then
+// fim_add_scalar as deconvolve_tiles does, dw.c:2131-2134).
+__global__ __launch_bounds__(STHR) void k_tile_extract(const float* __restrict__ im, int M, int N, int x0, int y0,
+                                                      int z0, int xm, int xn, int xp, float offset,
+                                                      float* __restrict__ out)
+{
+    const size_t n = (size_t)xm * xn * xp;
+    for (size_t i = (size_t)blockIdx.x * STHR + threadIdx.x; i < n; i += (size_t)gridDim.x * STHR) {
+        const int a = (int)(i % xm);
+        const size_t r = i / xm;
+        const int b = (int)(r % xn), c = (int)(r / xn);
+        float v = im[(size_t)(x0 + a) + (size_t)M * ((y0 + b) + (size_t)N * (z0 + c))];
+        if (offset > 0) v += offset;
+        out[i] = v;
+    }
+}
+
+// getWeight1d (tiling.c:169-195), float arithmetic as in the reference
+__device__ __forceinline__ float tile_w1d(float a, float b, float c, float d, int xi)
+{
+    const float x = (float)xi;
+    if (x < a || x > d) return 0.f;
+    if (x >= b && x <= c) return 1.f;
+    if (x >= a && x <= b) return (x - a + 1.f) / (b - a + 1.f);
+    return 1.f - (x - c) / (d - c + 1.f);
+}
+
+__device__ __forceinline__ float tile_w(const TileDev& t, int m, int n)
+{
+    const float wm = tile_w1d((float)t.xpos[0], (float)t.pos[0], (float)t.pos[1], (float)t.xpos[1], m);
+    const float wn = tile_w1d((float)t.xpos[2], (float)t.pos[2], (float)t.pos[3], (float)t.xpos[3], n);
+    return fminf(wm, wn);
+}
+
+// Blend (tiling_put_tile_raw, tiling.c:353-410, over all tiles in order):
+// out = sum_t (w_t / sum_s w_s) * tile_t, accumulated in float in tile
+// order from zero, each tile first shifted by -offset and clipped at 0 when
+// offset > 0 (dw.c:2141-2145).  One thread per (m, n) column of the image.
+__global__ __launch_bounds__(STHR) void k_tile_blend(const TileDev* __restrict__ tiles, int ntiles, int M, int N,
+                                                    int P, float offset, float* __restrict__ out)
+{
+    const size_t ncol = (size_t)M * N;
+    for (size_t col = (size_t)blockIdx.x * STHR + threadIdx.x; col < ncol; col += (size_t)gridDim.x * STHR) {
+        const int m = (int)(col % M), n = (int)(col / M);
+        float wsum = 0.f;
+        for (int t = 0; t < ntiles; ++t) wsum = __fadd_rn(wsum, tile_w(tiles[t], m, n));
+        for (int p = 0; p < P; ++p) out[col + ncol * p] = 0.f;
+        for (int t = 0; t < ntiles; ++t) {
+            const TileDev& T = tiles[t];
+            if (m < T.xpos[0] || m > T.xpos[1] || n < T.xpos[2] || n > T.xpos[3]) continue;
+            const float w = __fdiv_rn(tile_w(T, m, n), wsum);
+            const size_t base = (size_t)(m - T.xpos[0]) + (size_t)T.xsize[0] * (n - T.xpos[2]);
+            const size_t plane = (size_t)T.xsize[0] * T.xsize[1];
+            for (int p = T.xpos[4]; p <= T.xpos[5]; ++p) {
+                float s = T.out[base + plane * (p - T.xpos[4])];
+                if (offset > 0) {
+                    s += -offset;
+                    s = s < 0 ? 0.f : s;
+                }
+                // product and sum rounded separately, as the reference's
+                // buf[pos] += w*S (no fused multiply-add)
+                out[col + ncol * p] = __fadd_rn(out[col + ncol * p], __fmul_rn(w, s));
+            }
+        }
+    }
+}
+
+extern "C" hipError_t dwhip_launch_tile_extract(const float* im, int M, int N, int x0, int y0, int z0, int xm, int xn,
+                                               int xp, float offset, float* out, hipStream_t s)
+{
+    const size_t n = (size_t)xm * xn * xp;
+    const int grid = (int)std::min<size_t>((n + STHR - 1) / STHR, 4096);
+    hipLaunchKernelGGL(k_tile_extract, dim3(grid), dim3(STHR), 0, s, im, M, N, x0, y0, z0, xm, xn, xp, offset, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dwhip_launch_tile_blend(const TileDev* tiles, int ntiles, int M, int N, int P, float offset,
+                                             float* out, hipStream_t s)
+{
+    const size_t n = (size_t)M * N;
+    const int grid = (int)std::min<size_t>((n + STHR - 1) / STHR, 4096);
+    hipLaunchKernelGGL(k_tile_blend, dim3(grid), dim3(STHR), 0, s, tiles, ntiles, M, N, P, offset, out);
+    return hipGetLastError();
+}

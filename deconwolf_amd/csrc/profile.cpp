@@ -51,7 +51,7 @@ const char* dwhip_prof_kernel_name(int k)
 {
     static const char* names[DWHIP_PROF_KERNELS] = {
         "x_psf", "x_ones", "x_weights", "x_update", "x_ratio", "x_load", "x_store",
-        "y_fwd", "y_inv", "z_otf", "z_conv", "z_convj", "z_fwd", "z_inv",
+        "y_fwd", "y_inv", "z_otf", "z_conv", "z_convj", "z_fwd", "z_inv", "y_inv_img", "y_fwd_img",
     };
     return (k >= 0 && k < DWHIP_PROF_KERNELS) ? names[k] : "?";
 }

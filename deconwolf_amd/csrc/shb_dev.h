@@ -73,3 +73,14 @@ struct BatchDev {
     int njobs;
     int ntile[KIND_N];
 };
+
+// One tile of deconwolf's XY tiling (tiling.c): core box pos and the box
+// grown by the overlap, xpos (inclusive bounds x0 x1 y0 y1 z0 z1), and the
+// tile's deconvolved output (xsize volume, offset still added).
+struct TileDev {
+    int pos[6];
+    int xpos[6];
+    int xsize[3];
+    int pad;
+    const float* out;
+};

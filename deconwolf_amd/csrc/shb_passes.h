@@ -32,6 +32,7 @@
 
 #define XTHR DWHIP_XTHR
 #define CTHR DWHIP_CTHR
+#define ZTHR DWHIP_ZTHR
 
 __device__ __forceinline__ int find_job(const JobDev* __restrict__ jobs, int njobs, int kind, int t)
 {
@@ -113,6 +114,16 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         const int row0 = lt * J.xrows;
         const int nrows = min(J.xrows, ny * J.nz - row0);
         const int iter = (MODE == XM_UPDATE || MODE == XM_RATIO) ? J.st->iter : 0;
+        // the ratio is 0 outside the image box and only y < N, z < P rows of
+        // y are used: pass E reads and writes only those rows
+        if (MODE == XM_RATIO) {
+            const int z0 = row0 / ny, y0 = row0 - z0 * ny;
+            const int zl = (row0 + nrows - 1) / ny;
+            if (z0 >= J.P || (z0 == zl && y0 >= J.N)) {
+                if (tid == 0) J.part[lt] = 0.0;
+                continue;
+            }
+        }
         const bool inv = MODE == XM_RATIO || MODE == XM_WEIGHTS || MODE == XM_STORE ||
                          (MODE == XM_UPDATE && iter > 0);
         const bool fwd = MODE != XM_WEIGHTS && MODE != XM_STORE;
@@ -130,7 +141,12 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
 
         if (inv) {
             auto split_src = [&](int pen, int i) {
-                const float2* crow = C + (size_t)(row0 + pen) * hcs;
+                const int row = row0 + pen;
+                if (MODE == XM_RATIO) {
+                    const int zz = row / ny;
+                    if (zz >= J.P || row - zz * ny >= J.N) return make_float2(0.f, 0.f);
+                }
+                const float2* crow = C + (size_t)row * hcs;
                 return c2r_split(crow[i], crow[h - i], J.rtw[i]);
             };
             fft_tile<DWHIP_XMAXE, +1, false, false, true>(J.pxi, g, tid, lds, J.tw, split_src, lds_dst);
@@ -275,6 +291,10 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
             for (int w = tid; w < nrows * hp; w += XTHR) {
                 const int b = (int)fdiv((uint32_t)w, hpmagic);
                 const int k = w - b * hp;
+                if (MODE == XM_RATIO) {
+                    const int row = row0 + b, zz = row / ny;
+                    if (zz >= J.P || row - zz * ny >= J.N) continue;
+                }
                 const float2* L = lds + b * h;
                 float2* crow = dst + (size_t)(row0 + b) * hcs;
                 if (k == 0) {
@@ -314,12 +334,22 @@ struct OtfOp {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(CTHR) void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
+constexpr int col_threads()
+{
+    return cmode_is_y(MODE) ? CTHR : ZTHR;
+}
+
+// Y tiles (512 threads, up to 80 KiB of LDS) fit two workgroups per CU
+// when the kernel stays within 128 VGPRs.
+template <int MODE>
+__global__ __launch_bounds__(col_threads<MODE>()) __attribute__((amdgpu_waves_per_eu(cmode_is_y(MODE) ? 4 : 1)))
+void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
 {
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
-    constexpr int kind = (MODE == CM_YFWD || MODE == CM_YINV) ? KIND_Y : KIND_Z;
-    static_assert(MODE >= CM_YFWD && MODE <= CM_ZINV, "column mode");
+    constexpr int kind = cmode_is_y(MODE) ? KIND_Y : KIND_Z;
+    constexpr int NT = col_threads<MODE>();
+    static_assert(MODE >= 0 && MODE < DWHIP_N_CMODES, "column mode");
     const int total = bd->ntile[kind];
     const int njobs = bd->njobs;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -329,6 +359,8 @@ __global__ __launch_bounds__(CTHR) void k_colpass(const BatchDev* __restrict__ b
         const int B = kind == KIND_Y ? J.ycols : J.zcols;
         const int nct = (J.hc + B - 1) / B;
         const int outer = lt / nct;
+        // Y passes of the image chain touch only slabs z < P
+        if ((MODE == CM_YINV_IMG || MODE == CM_YFWD_IMG) && outer >= J.P) continue;
         const int kx0 = (lt - outer * nct) * B;
         const int nb = min(B, J.hc - kx0);   // columns past hc are row padding, never touched
         const size_t stride = kind == KIND_Y ? (size_t)hcs : (size_t)hcs * ny;
@@ -339,16 +371,21 @@ __global__ __launch_bounds__(CTHR) void k_colpass(const BatchDev* __restrict__ b
         g.n = kind == KIND_Y ? ny : J.nz;
         g.npen = B;
         g.pstride = 0;
-        g.nthr = CTHR;
+        g.nthr = NT;
         g.npmagic = fdiv_magic((uint32_t)B);
+        // elements read (i < nin) and written (i < nout) along the pencil
+        const int nin = (MODE == CM_YFWD_IMG) ? J.N : (MODE == CM_ZCONVJ) ? J.P : g.n;
+        const int nout = (MODE == CM_YINV_IMG) ? J.N : (MODE == CM_ZCONV) ? J.P : g.n;
 
-        auto gsrc = [&](int pen, int i) { return pen < nb ? src[i * stride + pen] : make_float2(0.f, 0.f); };
-        auto gdst = [&](int pen, int i, float2 v) {
-            if (pen < nb) dst[i * stride + pen] = v;
+        auto gsrc = [&](int pen, int i) {
+            return (pen < nb && i < nin) ? src[i * stride + pen] : make_float2(0.f, 0.f);
         };
-        if (MODE == CM_YFWD) {
+        auto gdst = [&](int pen, int i, float2 v) {
+            if (pen < nb && i < nout) dst[i * stride + pen] = v;
+        };
+        if (MODE == CM_YFWD || MODE == CM_YFWD_IMG) {
             fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pyf, g, tid, lds, J.tw, gsrc, gdst);
-        } else if (MODE == CM_YINV) {
+        } else if (MODE == CM_YINV || MODE == CM_YINV_IMG) {
             fft_tile<DWHIP_CMAXE, +1, true, false, false>(J.pyi, g, tid, lds, J.tw, gsrc, gdst);
         } else if (MODE == CM_ZFWD) {
             fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pzf, g, tid, lds, J.tw, gsrc, gdst);
@@ -368,7 +405,6 @@ __global__ __launch_bounds__(CTHR) void k_colpass(const BatchDev* __restrict__ b
         __syncthreads();
     }
 }
-
 
 // Dynamic LDS above 64 KiB must be allowed per kernel (gfx950 has 160 KiB
 // per workgroup).  The limit asked for leaves 1 KiB for static LDS.

@@ -72,3 +72,58 @@ def make(name):
     ps = fim_normalize_sum1(ps)
     im += np.float32(s.offset)
     return np.ascontiguousarray(im), np.ascontiguousarray(ps), s, psf.shape
+
+
+# ---------------------------------------------------------------------------
+# c4: mixed-size request stream
+# ---------------------------------------------------------------------------
+C4 = dict(xy=(256, 384, 512, 768, 1024), z=(20, 40, 60, 80), lambdas=(460.0, 520.0, 600.0, 690.0),
+          iters=(20, 50, 100), psf=(181, 181, 79), bg=120.0, blob_density=2000.0 / (1024 * 1024 * 60))
+
+
+def describe_stream(n, rate):
+    return ("c4 stream: %d jobs, Poisson arrivals at %g jobs/s, XY uniform over {256,384,512,768,1024}^2, "
+            "Z uniform over {20,40,60,80}, Gaussian blobs at the c2 density, background 120, Born-Wolf PSF "
+            "181x181x79 at 460/520/600/690 nm (NA 1.40), iterations uniform over {20,50,100}" % (n, rate))
+
+
+def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
+    """n jobs of the c4 stream: list of dicts with im, psf (torch tensors on
+    device, prepared like dw_run), opts, arrival (s).  Blurring runs on the
+    GPU through torch.fft (input generation only)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    xy = xy or C4["xy"]
+    psfs = {}
+    jobs = []
+    t = 0.0
+    pM, pN, pP = C4["psf"]
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        M = N = int(rng.choice(xy))
+        P = int(rng.choice(C4["z"]))
+        lam = float(rng.choice(C4["lambdas"]))
+        iters = int(rng.choice(C4["iters"]))
+        if lam not in psfs:
+            psfs[lam] = psf_bw.bw_psf(pM, pN, pP, lam, NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=200.0)
+        psf = psfs[lam]
+        nb = max(1, int(round(C4["blob_density"] * M * N * P)))
+        V = synth.gaussian_blobs((P, N, M), nb, sigma=(1.0, 3.0), amp=(200.0, 4000.0), rng=rng)
+        full = (P + pP - 1, N + pN - 1, M + pM - 1)
+        a = torch.as_tensor(V.astype(np.float32), device=device)
+        b = torch.as_tensor(psf, device=device)
+        B = torch.fft.irfftn(torch.fft.rfftn(a, full) * torch.fft.rfftn(b, full), full)
+        cz, cy, cx = (pP - 1) // 2, (pN - 1) // 2, (pM - 1) // 2
+        B = B[cz:cz + P, cy:cy + N, cx:cx + M].clamp_min(0) + C4["bg"]
+        g = torch.Generator(device=device)
+        g.manual_seed(int(seed * 100003 + i))
+        im = torch.poisson(B, generator=g).float().contiguous()
+        del a, B
+        s = DwOpts(nIter=iters, iter_type=2)
+        ps = fim_normalize_sum1(psf)
+        ps = psf_autocrop(ps, (P, N, M), s)
+        ps = fim_normalize_sum1(ps)
+        im += float(s.offset)
+        jobs.append(dict(im=im, psf=torch.as_tensor(ps, device=device).contiguous(), opts=s, arrival=t,
+                         shape=(M, N, P), iters=iters, lam=lam))
+    return jobs

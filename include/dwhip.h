@@ -186,13 +186,42 @@ int dwhip_engine_status(const dwhip_engine *engine, int *queued, int *active, si
 int dwhip_engine_result(dwhip_engine *engine, int64_t tag, dwhip_stats *stats, int release);
 
 /*
+ * XY tiling, deconwolf's tiling.c and the tile loop of deconvolve_tiles
+ * (src/dw.c:2000-2229), with the image and tiles in device memory instead
+ * of raw temporary files.  pos is a tile's core box and xpos the box grown
+ * by the overlap and clamped to the image (inclusive bounds x0 x1 y0 y1 z0
+ * z1); xsize = xpos extents.
+ */
+typedef struct {
+    int64_t pos[6];
+    int64_t xpos[6];
+    int64_t xsize[3];
+} dwhip_tile;
+
+/* tiling_create (src/tiling.c:70-130): ntiles receives the count; tiles
+ * (may be NULL to ask for the count) receives them, x-divisions outer. */
+int dwhip_tiling_create(int64_t M, int64_t N, int64_t P, int64_t max_size, int64_t overlap,
+                        dwhip_tile *tiles, int max_tiles, int *ntiles);
+/* Copies tile t's grown box out of im (M*N*P) into out (xsize volume) and
+ * adds offset when offset > 0 (src/dw.c:2113,2131-2134). */
+int dwhip_tile_extract_device(const float *im, int64_t M, int64_t N, int64_t P, const dwhip_tile *t,
+                              float offset, float *out, void *stream);
+/* Blends deconvolved tiles (outs[i]: tile i's method output, offset still
+ * added) into out (M*N*P): tile outputs get -offset and are clipped at 0
+ * when offset > 0 (src/dw.c:2141-2145), then out = sum over tiles, in
+ * order, of tile_getWeight / tiling_getWeights times the tile
+ * (src/tiling.c:197-223,353-410). */
+int dwhip_tiles_blend_device(const dwhip_tile *tiles, const float *const *outs, int ntiles, int64_t M,
+                             int64_t N, int64_t P, float offset, float *out, void *stream);
+
+/*
  * Kernel timing for measurement.  While enabled, every pass launch is
  * bracketed by HIP events on the stream it runs on and its algorithmic
  * bytes (each operand read or written once) are added up per kernel.
- * dwhip_profile_enable(1) clears the totals.  Kernel ids 0..13 name the
+ * dwhip_profile_enable(1) clears the totals.  Kernel ids 0..15 name the
  * X-pass modes (x_psf, x_ones, x_weights, x_update, x_ratio, x_load,
  * x_store) and column modes (y_fwd, y_inv, z_otf, z_conv, z_convj, z_fwd,
- * z_inv).  No counterpart in deconwolf (its timing is the --times flag,
+ * z_inv, y_inv_img, y_fwd_img).  No counterpart in deconwolf (its timing is the --times flag,
  * src/dw.c:2233-2319).
  */
 int dwhip_profile_enable(int on);
