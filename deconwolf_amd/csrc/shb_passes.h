@@ -333,6 +333,18 @@ struct OtfOp {
     }
 };
 
+// Workgroup b runs on XCD b mod 8 (dispatch is round robin over the 8 XCDs,
+// each with its own L2).  Column tile t covers B adjacent kx columns, so
+// tiles t and t+1 share the 128-byte lines of a row when B*8 < 128: map
+// consecutive workgroups of one XCD to consecutive tiles, so that both
+// halves of a line are read through the same L2.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_tile(int b, int G)
+{
+    const int q = G >> 3, r = G & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
 template <int MODE>
 constexpr int col_threads()
 {
@@ -352,7 +364,9 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
     static_assert(MODE >= 0 && MODE < DWHIP_N_CMODES, "column mode");
     const int total = bd->ntile[kind];
     const int njobs = bd->njobs;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    for (int t0 = blockIdx.x; t0 < total; t0 += gridDim.x) {
+        // XCD-aware order when the grid covers the tiles exactly once
+        const int t = (kind == KIND_Y && (int)gridDim.x == total) ? xcd_tile(t0, total) : t0;
         const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
         const int lt = t - J.tile0[kind];
         const int hcs = J.hcs, ny = J.ny;

@@ -181,3 +181,15 @@ def test_device_entry_matches_host_entry():
     b = dw.deconvolve_shb_hip_device(torch.from_numpy(im).to(DEV), torch.from_numpy(k).to(DEV), s2)
     torch.cuda.synchronize()
     assert np.array_equal(a, b.cpu().numpy())
+
+
+def test_shb_lp_start_matches_oracle():
+    """DW_START_LP: the image smoothed with sigma 8 (fim_gsmooth) inserted as
+    the start (method_shb.c:278-287)."""
+    from deconwolf_amd import filters
+    im, k = case((10, 24, 30), (5, 7, 7), seed=11)
+    s = _opts(6, start_condition=dw.START_LP)
+    out = dw.deconvolve_shb_hip(im, k, s)
+    lp = filters.gsmooth(im, 8.0)
+    o32, _, _, _ = po.shb(im, k, niter=6, start=po.START_LP, start_image=lp, work=s.stats["work"])
+    assert rel_l2(out, o32) <= 1e-5
