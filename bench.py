@@ -57,6 +57,65 @@ def run_cpu_baseline(npz, iters_full, threads, timeout_s):
     return d
 
 
+def _pct(xs, p):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(round(p / 100.0 * (len(xs) - 1))))]
+
+
+def run_stream(n, rate):
+    """The c4 stream served by the batched engine and, for comparison, by
+    the plain per-image path (FIFO, one job at a time), both against the
+    same arrival times; inputs are generated and resident first."""
+    import torch
+    import deconwolf_amd as dw
+    from deconwolf_amd import workloads
+    from deconwolf_amd.serve import Engine, serve
+
+    t = time.time()
+    jobs = workloads.make_stream(n, rate)
+    torch.cuda.synchronize()
+    t_gen = time.time() - t
+    work = sum(j["shape"][0] * j["shape"][1] * j["shape"][2] * j["iters"] for j in jobs)
+    arrivals = [j["arrival"] for j in jobs]
+
+    def opts(j):
+        return dw.DwOpts(nIter=j["iters"], iter_type=dw.ITER_FIXED)
+
+    # plain per-image, FIFO
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat_p, last = [], 0.0
+    for j in jobs:
+        now = time.perf_counter() - t0
+        if j["arrival"] > now:
+            time.sleep(j["arrival"] - now)
+        dw.deconvolve_shb_hip_device(j["im"], j["psf"], opts(j))
+        torch.cuda.synchronize()
+        last = time.perf_counter() - t0
+        lat_p.append(last - j["arrival"])
+    plain_s = last - arrivals[0]
+
+    free, total = torch.cuda.mem_get_info()
+    budget = max(4.0, 0.8 * free / (1 << 30))
+    eng = Engine(hbm_budget_gb=budget, max_active=64)
+    reqs = [(i, j["im"], j["psf"], opts(j)) for i, j in enumerate(jobs)]
+    times = serve(eng, reqs, arrivals)
+    eng.close()
+    lat_s = [d - a for a, d in times.values()]
+    served_s = max(d for _, d in times.values()) - arrivals[0]
+    return {
+        "workload": workloads.describe_stream(n, rate),
+        "voxel_iters": work,
+        "served_value": work / served_s,
+        "plain_value": work / plain_s,
+        "served_vs_plain": plain_s / served_s,
+        "served_p50_ms": 1e3 * _pct(lat_s, 50), "served_p99_ms": 1e3 * _pct(lat_s, 99),
+        "plain_p50_ms": 1e3 * _pct(lat_p, 50), "plain_p99_ms": 1e3 * _pct(lat_p, 99),
+        "engine_hbm_budget_gb": budget,
+        "input_gen_s": t_gen,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +124,24 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-timeout", type=int, default=240)
+    ap.add_argument("--stream", type=int, default=12, metavar="JOBS",
+                    help="jobs of the mixed-size c4 request stream run after the c2 steps, served (batched "
+                         "engine) and plain (one at a time); 0 skips it")
+    ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate of the stream, jobs/s")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only copy a GIB-sized float32 buffer on the GPU 3 times (the known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
     a = ap.parse_args()
+    if a.copy_calibration > 0:
+        import torch
+        n = int(a.copy_calibration * (1 << 30)) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 3}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs would shard one per GPU "
               "with no communication); run with --gpus 1", file=sys.stderr)
@@ -134,6 +210,10 @@ def main():
     def pct(p):
         return lat_sorted[min(len(lat_sorted) - 1, int(round(p / 100.0 * (len(lat_sorted) - 1))))]
 
+    served = None
+    if a.stream > 0:
+        served = run_stream(a.stream, a.rate)
+
     res = {
         "metric": METRIC,
         "value": value,
@@ -171,6 +251,7 @@ def main():
             "all_passes_GBps": (all_by / 1e9) / (all_ms / 1e3),
         },
         "cpu_baseline": cpu,
+        "served": served,
         "kernels": kernels,
         "input_gen_s": t_gen,
     }
