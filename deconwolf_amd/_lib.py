@@ -18,6 +18,10 @@ ERRORS = {
 ITER_ABS, ITER_REL, ITER_FIXED = 0, 1, 2
 METRIC_MSE, METRIC_IDIV = 0, 1
 START_FLAT, START_LP, START_IDENTITY = 0, 1, 2
+# dwhip_params.method
+DWHIP_METHOD_SHB, DWHIP_METHOD_RL = 0, 1
+# dw_method (src/dw.h:52-60), the values of DwOpts.method
+METHOD_RL, METHOD_ID, METHOD_SHB = 0, 1, 2
 
 
 class DwhipParams(ctypes.Structure):
@@ -37,6 +41,7 @@ class DwhipParams(ctypes.Structure):
         ("start_image", ctypes.c_void_p),
         ("work", ctypes.c_int64 * 3),
         ("verbosity", ctypes.c_int),
+        ("method", ctypes.c_int),
     ]
 
 
@@ -85,6 +90,10 @@ SIGNATURES = [
                              ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats)]),
     ("dwhip_shb_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                                     ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
+    ("dwhip_rl_run", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                            ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats)]),
+    ("dwhip_rl_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                                   ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
     ("dwhip_fft3_device", _int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
     ("dwhip_engine_config_default", None, [ctypes.POINTER(EngineConfig)]),
     ("dwhip_engine_create", _int, [ctypes.POINTER(EngineConfig), _vp, ctypes.POINTER(_vp)]),

@@ -1,11 +1,12 @@
 /*
  * deconwolf method module: the scaled-heavy-ball loop of deconvolve_shb
- * (src/method_shb.c:77-424) run on an MI355X by libdwhip.
+ * (src/method_shb.c:77-424) and the Richardson-Lucy loop of deconvolve_rl
+ * (src/method_rl.c:105-400) run on an MI355X by libdwhip.
  *
- * Contract, as for deconvolve_shb:
+ * Contract, as for deconvolve_shb and deconvolve_rl:
  *   - im is borrowed, already has the offset added (src/dw.c:2835-2838);
- *   - psf is owned and freed here (method_shb.c:103,191), normalised to
- *     sum 1 and autocropped by dw_run;
+ *   - psf is owned and freed here (method_shb.c:103,191; method_rl.c:221),
+ *     normalised to sum 1 and autocropped by dw_run;
  *   - the return value is a fresh M*N*P buffer from fim_malloc, released
  *     by the caller with fim_free;
  *   - s->bg receives the resolved background when s->bg_auto is set
@@ -24,15 +25,17 @@
 
 #include "method_shb_hip.h"
 #include "method_shb.h"
+#include "method_rl.h"
 #include "fim.h"
 #include "dwhip.h"
 #include "dwhip_adapter.h"
 
-float *deconvolve_shb_hip(float *restrict im, const int64_t M, const int64_t N, const int64_t P,
-                          float *restrict psf, const int64_t pM, const int64_t pN, const int64_t pP,
-                          dw_opts *s)
+static float *run_hip(int rl, float *restrict im, const int64_t M, const int64_t N, const int64_t P,
+                      float *restrict psf, const int64_t pM, const int64_t pN, const int64_t pP,
+                      dw_opts *s)
 {
-    if (s->nIter == 0 && s->iter_type == DW_ITER_FIXED) {
+    const char *tag = rl ? "rl_hip" : "shb_hip";
+    if (s->nIter == 0) {
         fim_free(psf);
         return fim_copy(im, M * N * P);
     }
@@ -57,8 +60,10 @@ float *deconvolve_shb_hip(float *restrict im, const int64_t M, const int64_t N, 
 
     int64_t work[3], ref[3];
     if (dwhip_work_size(M, N, P, pM, pN, pP, s->borderQuality, work, ref) == DWHIP_ERR_SIZE) {
-        fprintf(s->log, "shb_hip: job [%" PRId64 "x%" PRId64 "x%" PRId64 "] not supported on the GPU, "
-                "using the CPU method\n", ref[0], ref[1], ref[2]);
+        fprintf(s->log, "%s: job [%" PRId64 "x%" PRId64 "x%" PRId64 "] not supported on the GPU, "
+                "using the CPU method\n", tag, ref[0], ref[1], ref[2]);
+        if (rl)
+            return deconvolve_rl(im, M, N, P, psf, pM, pN, pP, s); /* frees psf */
         return deconvolve_shb(im, M, N, P, psf, pM, pN, pP, s); /* frees psf */
     }
 
@@ -83,14 +88,15 @@ float *deconvolve_shb_hip(float *restrict im, const int64_t M, const int64_t N, 
             "job: [%" PRId64 "x%" PRId64 "x%" PRId64 "] (transformed as %" PRId64 "x%" PRId64 "x%" PRId64 ")\n",
             M, N, P, pM, pN, pP, ref[0], ref[1], ref[2], work[0], work[1], work[2]);
 
-    const int rc = dwhip_shb_run(im, M, N, P, psf, pM, pN, pP, &p, out, &st);
+    const int rc = rl ? dwhip_rl_run(im, M, N, P, psf, pM, pN, pP, &p, out, &st)
+                      : dwhip_shb_run(im, M, N, P, psf, pM, pN, pP, &p, out, &st);
     fim_free(psf);
     if (im_lp != NULL)
         fim_free(im_lp);
     if (rc != DWHIP_OK) {
-        fprintf(stderr, "shb_hip: %s\n", dwhip_strerror(rc));
+        fprintf(stderr, "%s: %s\n", tag, dwhip_strerror(rc));
         if (s->log != NULL)
-            fprintf(s->log, "shb_hip: %s\n", dwhip_strerror(rc));
+            fprintf(s->log, "%s: %s\n", tag, dwhip_strerror(rc));
         exit(EXIT_FAILURE);
     }
     if (s->bg_auto) {
@@ -99,7 +105,21 @@ float *deconvolve_shb_hip(float *restrict im, const int64_t M, const int64_t N, 
     }
     dwhip_adapter_log_iterations(s->log, &v, trace, st.iterations);
     if (s->verbosity > 0)
-        printf("shb_hip: %d iterations, %.1f ms\n", st.iterations, st.setup_ms + st.iter_ms);
+        printf("%s: %d iterations, %.1f ms\n", tag, st.iterations, st.setup_ms + st.iter_ms);
     free(trace);
     return out;
+}
+
+float *deconvolve_shb_hip(float *restrict im, const int64_t M, const int64_t N, const int64_t P,
+                          float *restrict psf, const int64_t pM, const int64_t pN, const int64_t pP,
+                          dw_opts *s)
+{
+    return run_hip(0, im, M, N, P, psf, pM, pN, pP, s);
+}
+
+float *deconvolve_rl_hip(float *restrict im, const int64_t M, const int64_t N, const int64_t P,
+                         float *restrict psf, const int64_t pM, const int64_t pN, const int64_t pP,
+                         dw_opts *s)
+{
+    return run_hip(1, im, M, N, P, psf, pM, pN, pP, s);
 }

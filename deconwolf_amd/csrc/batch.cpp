@@ -320,7 +320,8 @@ int dwhip_engine_submit(dwhip_engine* E, const float* im, int64_t M, int64_t N, 
     for (EngJob* j : E->active)
         if (j->tag == tag) return DWHIP_ERR_ARG;
     const bool fixed = prm->iter_type == DWHIP_ITER_FIXED;
-    if ((fixed && prm->niter < 1) || (!fixed && prm->maxiter < 1)) return DWHIP_ERR_ARG;
+    if (prm->niter < 1 || (!fixed && prm->maxiter < 1)) return DWHIP_ERR_ARG;
+    if (prm->method != DWHIP_METHOD_SHB && prm->method != DWHIP_METHOD_RL) return DWHIP_ERR_ARG;
     EngJob* j = new EngJob();
     j->prm = *prm;
     j->tag = tag;
@@ -353,7 +354,7 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
     std::vector<EngJob*> fh;
     for (EngJob* j : E->active) {
         const bool fixed = j->prm.iter_type == DWHIP_ITER_FIXED;
-        if (!fixed || j->j.host_iter + 1 < j->prm.niter) fh.push_back(j);
+        if (!fixed || j->j.d.rl || j->j.host_iter + 1 < j->prm.niter) fh.push_back(j);
     }
     RCHK(table_build(&E->tae, E->active, s));
     RCHK(table_build(&E->tfh, fh, s));
@@ -382,7 +383,7 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         }
     }
     if (need_err) ECHK(hipStreamSynchronize(s));
-    // finished jobs: crop x_{n-1} into out
+    // finished jobs: crop the result into out
     std::vector<EngJob*> finished, still;
     for (size_t i = 0; i < E->active.size(); ++i) {
         EngJob* j = E->active[i];
@@ -395,7 +396,9 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
             fin = !keep_going(j, j->nrun);
         }
         if (fin) {
-            ECHK(dwhip_launch_crop(j->j.jd, 0, j->nrun - 1, j->out, 1024, s));
+            // SHB returns x_{n-1}; RL returns x_n, after one more update pass
+            if (j->j.d.rl) RCHK(dwhip_job_final_update(&j->j, j->j.bd, j->j.jd, s));
+            ECHK(dwhip_launch_crop(j->j.jd, 0, j->j.d.rl ? j->nrun : j->nrun - 1, j->out, 1024, s));
             finished.push_back(j);
         } else {
             still.push_back(j);

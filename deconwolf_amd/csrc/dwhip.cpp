@@ -72,6 +72,7 @@ void dwhip_params_default(dwhip_params* p, int niter)
     p->alphamax = 1.0f;
     p->metric = DWHIP_METRIC_IDIV;
     p->start = DWHIP_START_FLAT;
+    p->method = DWHIP_METHOD_SHB;
 }
 
 // ---------------------------------------------------------------------------
@@ -343,6 +344,7 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.bg_auto = prm->bg_auto;
     d.start_identity = prm->start != DWHIP_START_FLAT;
     d.niter = niter;
+    d.rl = prm->method == DWHIP_METHOD_RL;
     d.alphamax = prm->alphamax;
     d.bg_user = prm->bg;
     d.im = im_dev;
@@ -450,9 +452,9 @@ double dwhip_pass_bytes(int xkind, int mode, const JobDev& d, int iter)
     if (!xkind) {
         switch (mode) {
         case CM_ZCONV: return Vc + Vc + slabP;                 // read C, K; write z < P
-        case CM_ZCONVJ: return slabP + Vc + Vc;                // read z < P, K; write C
+        case CM_ZCONVJ: return d.rl ? 3 * Vc : slabP + Vc + Vc; // read z < P (RL: all), K; write C
         case CM_YINV_IMG: return slabP + imgrows;              // slabs z < P, write y < N
-        case CM_YFWD_IMG: return imgrows + slabP;
+        case CM_YFWD_IMG: return d.rl ? 2 * Vc : imgrows + slabP;
         default: return 2 * Vc;
         }
     }
@@ -460,8 +462,12 @@ double dwhip_pass_bytes(int xkind, int mode, const JobDev& d, int iter)
     case XM_PSF: return Vc + (double)d.pM * d.pN * d.pP * sizeof(float);
     case XM_ONES: return Vc;
     case XM_WEIGHTS: return Vc + V;
-    case XM_UPDATE: return iter == 0 ? 2 * V + Vc : 2 * Vc + (d.has_w ? 4 : 3) * V;
-    case XM_RATIO: return 2 * imgrows + img;
+    case XM_UPDATE:
+        // SHB reads x_{k-1}, x_{k-2} (and W), writes x_k; RL reads x_{k-1}
+        // (and W), writes x_k
+        if (iter == 0) return (d.rl ? 1 : 2) * V + Vc;
+        return 2 * Vc + ((d.rl ? 2 : 3) + (d.has_w ? 1 : 0)) * V;
+    case XM_RATIO: return d.rl ? imgrows + Vc + img : 2 * imgrows + img;
     case XM_LOAD: return Vc + V;
     case XM_STORE: return Vc + V;
     default: return 0;
@@ -541,6 +547,14 @@ int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s)
     return DWHIP_OK;
 }
 
+// x_n = max(H^T(ratio_{n-1}) * x_{n-1} * W, bg) of Richardson-Lucy after
+// its last round (method_rl.c:85-97, 357-366): the update half of pass A.
+// Its forward transform is written to C and not used.
+int dwhip_job_final_update(DwhipJob* j, const BatchDev* bd, const JobDev* jd, hipStream_t s)
+{
+    return dwhip_pass_x(XM_UPDATE, j, bd, jd, j->tiles.ntile[KIND_X], j->tiles.lds[KIND_X], s);
+}
+
 // dw_iterator_next for REL / ABS (src/dw.c:235-268), given the error of the
 // last two iterations run.
 static bool iterator_continue(const dwhip_params* p, int iter, float error, float lasterror)
@@ -579,7 +593,8 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
     if (rc == DWHIP_OK) {
         if (fixed) {
             for (int k = 0; k < prm->niter && rc == DWHIP_OK; ++k) {
-                rc = dwhip_job_round(&job, k + 1 < prm->niter, s);
+                // SHB returns x_{n-1}: the last adjoint half feeds nothing
+                rc = dwhip_job_round(&job, job.d.rl || k + 1 < prm->niter, s);
                 nrun = k + 1;
             }
         } else {
@@ -599,9 +614,11 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
             }
         }
     }
+    // RL returns x_n: one more update pass turns the last adjoint into x_n
+    if (rc == DWHIP_OK && nrun > 0 && job.d.rl) rc = dwhip_job_final_update(&job, job.bd, job.jd, s);
     hipEventRecord(e2, s);
     if (rc == DWHIP_OK && nrun > 0) {
-        hipError_t e = dwhip_launch_crop(job.jd, 0, nrun - 1, out, 1024, s);
+        hipError_t e = dwhip_launch_crop(job.jd, 0, job.d.rl ? nrun : nrun - 1, out, 1024, s);
         if (e != hipSuccess) rc = dwhip_hip_fail(e, "crop", __LINE__);
     }
     hipError_t e = hipStreamSynchronize(s);
@@ -648,12 +665,13 @@ static int check_args(const float* im, int64_t M, int64_t N, int64_t P, const fl
     if (prm->iter_type < DWHIP_ITER_ABS || prm->iter_type > DWHIP_ITER_FIXED) return DWHIP_ERR_ARG;
     if (prm->metric != DWHIP_METRIC_MSE && prm->metric != DWHIP_METRIC_IDIV) return DWHIP_ERR_ARG;
     if (prm->start < DWHIP_START_FLAT || prm->start > DWHIP_START_IDENTITY) return DWHIP_ERR_ARG;
-    if (prm->iter_type == DWHIP_ITER_FIXED && prm->niter < 0) return DWHIP_ERR_ARG;
+    if (prm->method != DWHIP_METHOD_SHB && prm->method != DWHIP_METHOD_RL) return DWHIP_ERR_ARG;
+    if (prm->niter < 0) return DWHIP_ERR_ARG;
     if (prm->iter_type != DWHIP_ITER_FIXED && prm->maxiter < 0) return DWHIP_ERR_ARG;
     return DWHIP_OK;
 }
 
-int dwhip_shb_run_device(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+static int run_device(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
                          int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, void* stream)
 {
     int rc = check_args(im, M, N, P, psf, pM, pN, pP, prm, out);
@@ -661,8 +679,8 @@ int dwhip_shb_run_device(const float* im, int64_t M, int64_t N, int64_t P, const
     rc = dwhip_init();
     if (rc != DWHIP_OK) return rc;
     hipStream_t s = (hipStream_t)stream;
-    if (prm->iter_type == DWHIP_ITER_FIXED && prm->niter == 0) {
-        // method_shb.c:101-105
+    if (prm->niter == 0) {  // any iter_type, as the reference tests nIter alone
+        // method_shb.c:101-105, method_rl.c:120-123
         HIPCHK(hipMemcpyAsync(out, im, (size_t)M * N * P * sizeof(float), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
         if (st) {
@@ -674,14 +692,14 @@ int dwhip_shb_run_device(const float* im, int64_t M, int64_t N, int64_t P, const
     return run_job(im, M, N, P, psf, pM, pN, pP, prm, out, st, s);
 }
 
-int dwhip_shb_run(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+static int run_host(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
                   int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st)
 {
     int rc = check_args(im, M, N, P, psf, pM, pN, pP, prm, out);
     if (rc != DWHIP_OK) return rc;
     const size_t n = (size_t)M * N * P, np = (size_t)pM * pN * pP;
-    if (prm->iter_type == DWHIP_ITER_FIXED && prm->niter == 0) {
-        memcpy(out, im, n * sizeof(float));  // method_shb.c:101-105
+    if (prm->niter == 0) {  // any iter_type, as the reference tests nIter alone
+        memcpy(out, im, n * sizeof(float));  // method_shb.c:101-105, method_rl.c:120-123
         if (st) {
             st->bg = prm->bg;
             st->iterations = 0;
@@ -718,6 +736,45 @@ int dwhip_shb_run(const float* im, int64_t M, int64_t N, int64_t P, const float*
     if (dstart) hipFree(dstart);
     if (s) hipStreamDestroy(s);
     return rc;
+}
+
+static dwhip_params with_method(const dwhip_params* prm, int method)
+{
+    dwhip_params p = *prm;
+    p.method = method;
+    return p;
+}
+
+int dwhip_shb_run_device(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                         int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, void* stream)
+{
+    if (!prm) return DWHIP_ERR_ARG;
+    const dwhip_params p = with_method(prm, DWHIP_METHOD_SHB);
+    return run_device(im, M, N, P, psf, pM, pN, pP, &p, out, st, stream);
+}
+
+int dwhip_rl_run_device(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                        int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, void* stream)
+{
+    if (!prm) return DWHIP_ERR_ARG;
+    const dwhip_params p = with_method(prm, DWHIP_METHOD_RL);
+    return run_device(im, M, N, P, psf, pM, pN, pP, &p, out, st, stream);
+}
+
+int dwhip_shb_run(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                  int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st)
+{
+    if (!prm) return DWHIP_ERR_ARG;
+    const dwhip_params p = with_method(prm, DWHIP_METHOD_SHB);
+    return run_host(im, M, N, P, psf, pM, pN, pP, &p, out, st);
+}
+
+int dwhip_rl_run(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                 int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st)
+{
+    if (!prm) return DWHIP_ERR_ARG;
+    const dwhip_params p = with_method(prm, DWHIP_METHOD_RL);
+    return run_host(im, M, N, P, psf, pM, pN, pP, &p, out, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -67,3 +67,4 @@ int dwhip_pass_c(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* 
 void dwhip_job_free(DwhipJob* j);
 int dwhip_job_setup(const DwhipJob* j, hipStream_t s);
 int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s);
+int dwhip_job_final_update(DwhipJob* j, const BatchDev* bd, const JobDev* jd, hipStream_t s);

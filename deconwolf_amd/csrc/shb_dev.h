@@ -46,6 +46,7 @@ struct JobDev {
     int bg_auto;
     int start_identity; // start from start_img (or im) inserted, instead of flat
     int niter;          // iterations requested
+    int rl;             // 1: Richardson-Lucy (method_rl.c), 0: SHB (method_shb.c)
     float alphamax;
     float bg_user;
     JobState* st;       // device-written state of this job

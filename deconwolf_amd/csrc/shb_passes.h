@@ -114,9 +114,10 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         const int row0 = lt * J.xrows;
         const int nrows = min(J.xrows, ny * J.nz - row0);
         const int iter = (MODE == XM_UPDATE || MODE == XM_RATIO) ? J.st->iter : 0;
-        // the ratio is 0 outside the image box and only y < N, z < P rows of
-        // y are used: pass E reads and writes only those rows
-        if (MODE == XM_RATIO) {
+        // the SHB ratio is 0 outside the image box and only y < N, z < P rows
+        // of y are used: pass E reads and writes only those rows (the RL
+        // ratio is 1e-6 outside the image, so RL writes every row)
+        if (MODE == XM_RATIO && !J.rl) {
             const int z0 = row0 / ny, y0 = row0 - z0 * ny;
             const int zl = (row0 + nrows - 1) / ny;
             if (z0 >= J.P || (z0 == zl && y0 >= J.N)) {
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                 if (inv) a[q] = lds[w0 + q * XTHR];
                 if (MODE == XM_UPDATE) {
                     b[q] = *reinterpret_cast<const float2*>(Ukm1 + vi[q]);
-                    c[q] = *reinterpret_cast<const float2*>(Uk + vi[q]);
+                    if (!J.rl) c[q] = *reinterpret_cast<const float2*>(Uk + vi[q]);
                     if (iter > 0 && J.has_w) wv[q] = *reinterpret_cast<const float2*>(J.W + vi[q]);
                 } else if (MODE == XM_LOAD) {
                     a[q] = *reinterpret_cast<const float2*>(J.U0 + vi[q]);
@@ -226,6 +227,24 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                     out.x = p1.x > sigma ? (float)(1.0 / p1.x) : 0.f;
                     out.y = p1.y > sigma ? (float)(1.0 / p1.y) : 0.f;
                     *reinterpret_cast<float2*>(J.W + vi[q]) = out;
+                } else if (MODE == XM_UPDATE && J.rl) {
+                    // Richardson-Lucy (method_rl.c:85-97, 357-366): f = x_{k-1},
+                    // x_k = x~ * (f * W), clamped to bg when bg > 0
+                    const float2 f = b[q];
+                    float2 xk = f;  // iteration 0: the start guess
+                    if (iter > 0) {
+                        const float2 xt = a[q];
+                        if (J.has_w)
+                            xk = make_float2(xt.x * (f.x * wv[q].x), xt.y * (f.y * wv[q].y));
+                        else
+                            xk = make_float2(xt.x * f.x, xt.y * f.y);
+                        if (bg > 0) {
+                            xk.x = xk.x < bg ? bg : xk.x;
+                            xk.y = xk.y < bg ? bg : xk.y;
+                        }
+                        *reinterpret_cast<float2*>(Uk + vi[q]) = xk;
+                    }
+                    out = xk;
                 } else if (MODE == XM_UPDATE) {
                     const float2 xkm1 = b[q];
                     float2 xk;
@@ -252,7 +271,10 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                     out = make_float2(shb_extrap(xk.x, xkm1.x, al, bg), shb_extrap(xk.y, xkm1.y, al, bg));
                 } else {  // XM_RATIO, method_shb.c:27-48 and getError dw.c:1436-1524
                     const float2 yv = a[q];
-                    float r[2] = {0.f, 0.f};
+                    // outside the image: 0 for SHB (method_shb.c:44), 1e-6 for RL
+                    // (method_rl.c:62)
+                    const float outside = J.rl ? 1e-6f : 0.f;
+                    float r[2] = {outside, outside};
                     if (yy[q] < J.N && zz[q] < J.P) {
 #pragma unroll
                         for (int e = 0; e < 2; ++e) {
@@ -266,8 +288,12 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                                 } else {
                                     acc += (yd - gd) * (yd - gd);
                                 }
-                                if (fabs(yd) < 1e-6) y = (float)copysign(1e-6, yd);
-                                r[e] = gv / y;
+                                if (J.rl) {
+                                    r[e] = y > 0 ? gv / y : bg;  // method_rl.c:52-60
+                                } else {
+                                    if (fabs(yd) < 1e-6) y = (float)copysign(1e-6, yd);
+                                    r[e] = gv / y;
+                                }
                             }
                         }
                     }
@@ -291,7 +317,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
             for (int w = tid; w < nrows * hp; w += XTHR) {
                 const int b = (int)fdiv((uint32_t)w, hpmagic);
                 const int k = w - b * hp;
-                if (MODE == XM_RATIO) {
+                if (MODE == XM_RATIO && !J.rl) {
                     const int row = row0 + b, zz = row / ny;
                     if (zz >= J.P || row - zz * ny >= J.N) continue;
                 }
@@ -373,8 +399,9 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         const int B = kind == KIND_Y ? J.ycols : J.zcols;
         const int nct = (J.hc + B - 1) / B;
         const int outer = lt / nct;
-        // Y passes of the image chain touch only slabs z < P
-        if ((MODE == CM_YINV_IMG || MODE == CM_YFWD_IMG) && outer >= J.P) continue;
+        // Y passes of the image chain touch only slabs z < P (the RL
+        // ratio is 1e-6 outside the image, so its adjoint is not pruned)
+        if ((MODE == CM_YINV_IMG || (MODE == CM_YFWD_IMG && !J.rl)) && outer >= J.P) continue;
         const int kx0 = (lt - outer * nct) * B;
         const int nb = min(B, J.hc - kx0);   // columns past hc are row padding, never touched
         const size_t stride = kind == KIND_Y ? (size_t)hcs : (size_t)hcs * ny;
@@ -388,7 +415,7 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         g.nthr = NT;
         g.npmagic = fdiv_magic((uint32_t)B);
         // elements read (i < nin) and written (i < nout) along the pencil
-        const int nin = (MODE == CM_YFWD_IMG) ? J.N : (MODE == CM_ZCONVJ) ? J.P : g.n;
+        const int nin = (MODE == CM_YFWD_IMG && !J.rl) ? J.N : (MODE == CM_ZCONVJ && !J.rl) ? J.P : g.n;
         const int nout = (MODE == CM_YINV_IMG) ? J.N : (MODE == CM_ZCONV) ? J.P : g.n;
 
         auto gsrc = [&](int pen, int i) {

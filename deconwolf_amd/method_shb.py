@@ -25,13 +25,15 @@ import numpy as np
 
 from . import _lib
 from ._lib import (DwhipParams, DwhipStats, check, lib, ITER_ABS, ITER_REL, ITER_FIXED,
-                   METRIC_MSE, METRIC_IDIV, START_FLAT, START_LP, START_IDENTITY)
+                   METRIC_MSE, METRIC_IDIV, START_FLAT, START_LP, START_IDENTITY,
+                   METHOD_RL, METHOD_ID, METHOD_SHB, DWHIP_METHOD_SHB, DWHIP_METHOD_RL)
 
 
 @dataclass
 class DwOpts:
-    """The dw_opts fields on the SHB path (src/dw.h:104-211)."""
+    """The dw_opts fields on the SHB and RL paths (src/dw.h:104-211)."""
 
+    method: int = METHOD_SHB          # dw.c:376 (dw_method values, dw.h:52-60)
     nIter: int = 1                    # dw.c:349
     iter_type: int = ITER_REL         # dw.c:361
     maxiter: int = 250                # dw.c:350
@@ -75,6 +77,12 @@ def _params(s: DwOpts) -> DwhipParams:
     for i in range(3):
         p.work[i] = int(w[i])
     p.verbosity = int(s.verbosity)
+    if int(s.method) == METHOD_SHB:
+        p.method = DWHIP_METHOD_SHB
+    elif int(s.method) == METHOD_RL:
+        p.method = DWHIP_METHOD_RL
+    else:
+        raise ValueError("method %r has no GPU path (SHB and RL do)" % (s.method,))
     return p
 
 
@@ -118,6 +126,17 @@ def deconvolve_shb_hip(im: np.ndarray, psf: np.ndarray, s: DwOpts) -> np.ndarray
     psf: (pP, pN, pM) float32 PSF, sum 1, autocropped.
     Returns a new (P, N, M) float32 array: x_{n-1} after n iterations
     (method_shb.c:385-413)."""
+    return _run_host("dwhip_shb_run", im, psf, s)
+
+
+def deconvolve_rl_hip(im: np.ndarray, psf: np.ndarray, s: DwOpts) -> np.ndarray:
+    """Richardson-Lucy on the GPU with the meaning of deconvolve_rl
+    (src/method_rl.c:105-400): same inputs as :func:`deconvolve_shb_hip`;
+    returns the last iterate x_n cropped to (P, N, M)."""
+    return _run_host("dwhip_rl_run", im, psf, s)
+
+
+def _run_host(entry, im, psf, s):
     im = np.ascontiguousarray(im, dtype=np.float32)
     psf = np.ascontiguousarray(psf, dtype=np.float32)
     if im.ndim != 3 or psf.ndim != 3:
@@ -134,9 +153,8 @@ def deconvolve_shb_hip(im: np.ndarray, psf: np.ndarray, s: DwOpts) -> np.ndarray
     st = DwhipStats()
     st.err_trace = err.ctypes.data
     st.err_trace_len = nerr
-    check(lib().dwhip_shb_run(im.ctypes.data, M, N, P, psf.ctypes.data, pM, pN, pP,
-                              ctypes.byref(p), out.ctypes.data, ctypes.byref(st)),
-          "dwhip_shb_run")
+    check(getattr(lib(), entry)(im.ctypes.data, M, N, P, psf.ctypes.data, pM, pN, pP,
+                                ctypes.byref(p), out.ctypes.data, ctypes.byref(st)), entry)
     _finish(s, st, err)
     return out
 
@@ -145,6 +163,16 @@ def deconvolve_shb_hip_device(im, psf, s: DwOpts, out=None, start_image=None):
     """Same as :func:`deconvolve_shb_hip` for torch tensors already on the
     GPU (float32, contiguous, shapes (P,N,M) / (pP,pN,pM)).  Work is queued
     on torch's current stream.  Returns `out` (allocated if None)."""
+    return _run_device("dwhip_shb_run_device", im, psf, s, out, start_image)
+
+
+def deconvolve_rl_hip_device(im, psf, s: DwOpts, out=None, start_image=None):
+    """:func:`deconvolve_rl_hip` for GPU tensors, as
+    :func:`deconvolve_shb_hip_device`."""
+    return _run_device("dwhip_rl_run_device", im, psf, s, out, start_image)
+
+
+def _run_device(entry, im, psf, s, out, start_image):
     import torch
     if not (im.is_cuda and psf.is_cuda):
         raise ValueError("im and psf must be GPU tensors")
@@ -168,9 +196,8 @@ def deconvolve_shb_hip_device(im, psf, s: DwOpts, out=None, start_image=None):
     st.err_trace = err.ctypes.data
     st.err_trace_len = nerr
     stream = torch.cuda.current_stream().cuda_stream
-    check(lib().dwhip_shb_run_device(im.data_ptr(), M, N, P, psf.data_ptr(), pM, pN, pP,
-                                     ctypes.byref(p), out.data_ptr(), ctypes.byref(st), stream),
-          "dwhip_shb_run_device")
+    check(getattr(lib(), entry)(im.data_ptr(), M, N, P, psf.data_ptr(), pM, pN, pP,
+                                ctypes.byref(p), out.data_ptr(), ctypes.byref(st), stream), entry)
     _finish(s, st, err)
     return out
 
@@ -254,4 +281,6 @@ def deconvolve(image, psf, iterations, bq=2, verbose=0, opts: Optional[DwOpts] =
     ps = fim_normalize_sum1(ps)
     if s.offset > 0:
         im += np.float32(s.offset)
+    if int(s.method) == METHOD_RL:
+        return deconvolve_rl_hip(im, ps, s)
     return deconvolve_shb_hip(im, ps, s)

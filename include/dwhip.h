@@ -41,6 +41,11 @@ extern "C" {
 #define DWHIP_METRIC_MSE 0
 #define DWHIP_METRIC_IDIV 1
 /* dw_start_condition (src/dw.h:64-72) */
+/* deconvolution method (dw_method, src/dw.h:52-60): the scaled heavy-ball
+ * loop of method_shb.c or plain Richardson-Lucy of method_rl.c */
+#define DWHIP_METHOD_SHB 0
+#define DWHIP_METHOD_RL 1
+
 #define DWHIP_START_FLAT 0
 #define DWHIP_START_LP 1
 #define DWHIP_START_IDENTITY 2
@@ -65,6 +70,8 @@ typedef struct {
     const float *start_image; /* M*N*P start for IDENTITY/LP, NULL = im (same memory space as im) */
     int64_t work[3];     /* transform size; zeros = automatic (see dwhip_work_size) */
     int verbosity;       /* 0 silent; > 0 prints sizes and timings to stdout */
+    int method;          /* DWHIP_METHOD_*: read by the engine and the tiled path;
+                            dwhip_shb_run* and dwhip_rl_run* set it themselves */
 } dwhip_params;
 
 /* Results besides the output volume. */
@@ -131,6 +138,23 @@ int dwhip_shb_run(const float *im, int64_t M, int64_t N, int64_t P, const float 
 int dwhip_shb_run_device(const float *im, int64_t M, int64_t N, int64_t P, const float *psf,
                          int64_t pM, int64_t pN, int64_t pP, const dwhip_params *params,
                          float *out, dwhip_stats *stats, void *stream);
+
+/*
+ * Richardson-Lucy deconvolution, host buffers: the drop-in for
+ * deconvolve_rl (src/method_rl.c:105-400), the other dw_function of the
+ * same shape.  Same arguments as dwhip_shb_run; out receives the last
+ * iterate x_n (no swap-back); positivity and alphamax are not read, x is
+ * clamped to bg when bg > 0 (method_rl.c:357-366); the ratio outside the
+ * image is 1e-6 and g/y <= 0 gives bg (method_rl.c:45-65).
+ */
+int dwhip_rl_run(const float *im, int64_t M, int64_t N, int64_t P, const float *psf, int64_t pM,
+                 int64_t pN, int64_t pP, const dwhip_params *params, float *out,
+                 dwhip_stats *stats);
+
+/* Same on device memory and a caller stream, as dwhip_shb_run_device. */
+int dwhip_rl_run_device(const float *im, int64_t M, int64_t N, int64_t P, const float *psf,
+                        int64_t pM, int64_t pN, int64_t pP, const dwhip_params *params,
+                        float *out, dwhip_stats *stats, void *stream);
 
 /*
  * FFT check entry (device memory): unnormalised 3-D R2C of an nx*ny*nz

@@ -60,7 +60,7 @@ def lib():
         L.oracle_cfft.argtypes = [dp, ctypes.c_int, ctypes.c_int]
         L.oracle_rfft3.argtypes = [dp, i64, i64, i64, dp]
         L.oracle_irfft3.argtypes = [dp, i64, i64, i64, dp]
-        for name in ("oracle_shb_f32", "oracle_shb_f64"):
+        for name in ("oracle_shb_f32", "oracle_shb_f64", "oracle_rl_f32", "oracle_rl_f64"):
             f = getattr(L, name)
             f.argtypes = [dp, i64, i64, i64, dp, i64, i64, i64,
                           ctypes.POINTER(ShbOpts), dp, dp, dp]
@@ -134,6 +134,16 @@ def make_opts(niter=10, iter_type=ITER_FIXED, maxiter=250, err_rel=0.02, err_abs
 def shb(im, psf, opts=None, dtype=np.float32, start_image=None, **kw):
     """Run the oracle.  im: (P, N, M) array, psf: (pP, pN, pM) array, both
     x fastest.  Returns (out, err_trace, bg, iterations_run)."""
+    return _run("shb", im, psf, opts, dtype, start_image, kw)
+
+
+def rl(im, psf, opts=None, dtype=np.float32, start_image=None, **kw):
+    """Richardson-Lucy (method_rl.c) on the same options as shb(); same
+    return value."""
+    return _run("rl", im, psf, opts, dtype, start_image, kw)
+
+
+def _run(method, im, psf, opts, dtype, start_image, kw):
     o = opts if opts is not None else make_opts(**kw)
     im_a = np.ascontiguousarray(im, dtype=dtype)
     psf_a = np.ascontiguousarray(psf, dtype=dtype)
@@ -147,12 +157,12 @@ def shb(im, psf, opts=None, dtype=np.float32, start_image=None, **kw):
     nmax = max(o.niter, o.maxiter, 1)
     err = np.zeros(nmax + 1, dtype=np.float32)
     stats = np.zeros(4, dtype=np.float64)
-    f = lib().oracle_shb_f32 if dtype == np.float32 else lib().oracle_shb_f64
+    f = getattr(lib(), "oracle_%s_%s" % (method, "f32" if dtype == np.float32 else "f64"))
     rc = f(_ptr(im_a), M, N, P, _ptr(psf_a), pM, pN, pP, ctypes.byref(o), _ptr(out),
            _ptr(err), _ptr(stats))
     del keep
     if rc != 0:
-        raise RuntimeError("oracle_shb failed with code %d" % rc)
+        raise RuntimeError("oracle_%s failed with code %d" % (method, rc))
     n = int(stats[1])
     global last_times
     last_times = (float(stats[2]), float(stats[3]))

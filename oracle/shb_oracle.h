@@ -1,6 +1,7 @@
 /*
  * SHB oracle: a CPU restatement of deconwolf's scaled-heavy-ball loop
- * (src/method_shb.c:77-424).  TEST INFRASTRUCTURE ONLY: used by tests/,
+ * (src/method_shb.c:77-424) and of its Richardson-Lucy method
+ * (src/method_rl.c:28-400).  TEST INFRASTRUCTURE ONLY: used by tests/,
  * the smoke check and the CPU-baseline leg of the benchmark as the
  * checker, never as the product.
  */
@@ -38,6 +39,14 @@ int oracle_shb_f32(const float *im, int64_t M, int64_t N, int64_t P, const float
 int oracle_shb_f64(const double *im, int64_t M, int64_t N, int64_t P, const double *psf,
                    int64_t pM, int64_t pN, int64_t pP, const oracle_shb_opts *o, double *out,
                    float *err_trace, double *stats);
+/* Richardson-Lucy (src/method_rl.c:105-400) on the same options; positivity
+ * and alphamax are not read. */
+int oracle_rl_f32(const float *im, int64_t M, int64_t N, int64_t P, const float *psf,
+                  int64_t pM, int64_t pN, int64_t pP, const oracle_shb_opts *o, float *out,
+                  float *err_trace, double *stats);
+int oracle_rl_f64(const double *im, int64_t M, int64_t N, int64_t P, const double *psf,
+                  int64_t pM, int64_t pN, int64_t pP, const oracle_shb_opts *o, double *out,
+                  float *err_trace, double *stats);
 int oracle_shb_setup_f32(const float *im, int64_t M, int64_t N, int64_t P, const float *psf,
                          int64_t pM, int64_t pN, int64_t pP, const oracle_shb_opts *o,
                          float *cK, float *W, float *x0, double *stats);
