@@ -18,6 +18,8 @@ ERRORS = {
 ITER_ABS, ITER_REL, ITER_FIXED = 0, 1, 2
 METRIC_MSE, METRIC_IDIV = 0, 1
 START_FLAT, START_LP, START_IDENTITY = 0, 1, 2
+# dwhip_to_u16_device modes
+U16_ROUND, U16_TRUNC, U16_RINT = 0, 1, 2
 # dwhip_params.method
 DWHIP_METHOD_SHB, DWHIP_METHOD_RL = 0, 1
 # dw_method (src/dw.h:52-60), the values of DwOpts.method
@@ -94,6 +96,9 @@ SIGNATURES = [
                             ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats)]),
     ("dwhip_rl_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                                    ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
+    ("dwhip_output_stats_device", _int, [_vp, _i64, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_int64), _vp]),
+    ("dwhip_to_u16_device", _int, [_vp, _i64, ctypes.c_float, _int, _vp, _vp]),
     ("dwhip_fft3_device", _int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
     ("dwhip_engine_config_default", None, [ctypes.POINTER(EngineConfig)]),
     ("dwhip_engine_create", _int, [ctypes.POINTER(EngineConfig), _vp, ctypes.POINTER(_vp)]),

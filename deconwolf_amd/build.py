@@ -44,6 +44,7 @@ def units():
     for m in range(N_CMODES):
         u.append(("k_colpass.hip", ["-DCMODE=%d" % m], "k_colpass_%d.o" % m))
     u.append(("k_misc.hip", [], "k_misc.o"))
+    u.append(("k_output.hip", [], "k_output.o"))
     for src in ("dwhip.cpp", "profile.cpp", "batch.cpp", "tiling.cpp"):
         if os.path.exists(os.path.join(CSRC, src)):
             u.append((src, [], src.replace(".cpp", ".o")))

@@ -48,6 +48,8 @@ class DwOpts:
     start_condition: int = START_FLAT  # dw.c:410
     offset: float = 5.0               # dw.c:394
     xycropfactor: float = 0.001       # dw.c:377
+    outFormat: int = 16               # dw.c:383 (16: uint16 with scaling, 32: float32)
+    scaling: float = -1.0             # dw.c:384 (<= 0: automatic)
     verbosity: int = 0
     # transform size; None = automatic (smallest 2,3,5,7-smooth size >= the
     # reference work size for borderQuality 2)

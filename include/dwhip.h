@@ -157,6 +157,32 @@ int dwhip_rl_run_device(const float *im, int64_t M, int64_t N, int64_t P, const 
                         float *out, dwhip_stats *stats, void *stream);
 
 /*
+ * Output path of dw_run (src/dw.c:2862-2904) on the device.
+ *
+ * dwhip_output_stats_device: largest value of v (NaN skipped), from which
+ * the 16-bit scaling is (2^16-1)/max (scaling_for_u16, src/dw.c:98-103;
+ * 65535/max for tiled runs, src/dw.c:2166-2172), and the number of voxels
+ * equal to bg (get_nbg, src/dw.c:2406-2418, logged as the percentage of
+ * pixels at the background level).  Synchronises `stream`.
+ *
+ * dwhip_to_u16_device: out[i] = the uint16 that deconwolf's writer `mode`
+ * stores for v[i] with this scaling:
+ *   DWHIP_U16_ROUND  TIFF writer fim_tiff_write_opt (src/fim_tiff.c:1138-1150)
+ *   DWHIP_U16_TRUNC  .npy writer fim_imwrite_u16 (src/fim.c:5142-5147) and the
+ *                    tiled TIFF writer (src/fim_tiff.c:819-822)
+ *   DWHIP_U16_RINT   tiled .npy writer raw_to_npio (src/dw.c:187-190)
+ * Values the C conversions leave undefined (outside [0, 65536), NaN)
+ * saturate to [0, 65535], NaN to 0.  Queued on `stream`.
+ */
+#define DWHIP_U16_ROUND 0
+#define DWHIP_U16_TRUNC 1
+#define DWHIP_U16_RINT 2
+int dwhip_output_stats_device(const float *v, int64_t n, float bg, float *vmax, int64_t *n_bg,
+                              void *stream);
+int dwhip_to_u16_device(const float *v, int64_t n, float scaling, int mode, uint16_t *out,
+                        void *stream);
+
+/*
  * FFT check entry (device memory): unnormalised 3-D R2C of an nx*ny*nz
  * real volume (nx even) into (nx/2+1)*ny*nz complex, interleaved, the
  * layout of fftwf_plan_dft_r2c_3d(nz, ny, nx) that src/fft.c:236-290
