@@ -266,6 +266,7 @@ __global__ __launch_bounds__(STHR) void k_tile_extract(const float* __restrict__
 // getWeight1d (tiling.c:169-195), float arithmetic as in the reference
 __device__ __forceinline__ float tile_w1d(float a, float b, float c, float d, int xi)
 {
+#pragma clang fp contract(off)
     const float x = (float)xi;
     if (x < a || x > d) return 0.f;
     if (x >= b && x <= c) return 1.f;
@@ -287,16 +288,19 @@ __device__ __forceinline__ float tile_w(const TileDev& t, int m, int n)
 __global__ __launch_bounds__(STHR) void k_tile_blend(const TileDev* __restrict__ tiles, int ntiles, int M, int N,
                                                     int P, float offset, float* __restrict__ out)
 {
+    // products and sums rounded separately, as the reference's
+    // buf[pos] += w*S compiled for x86-64 without FMA (no contraction)
+#pragma clang fp contract(off)
     const size_t ncol = (size_t)M * N;
     for (size_t col = (size_t)blockIdx.x * STHR + threadIdx.x; col < ncol; col += (size_t)gridDim.x * STHR) {
         const int m = (int)(col % M), n = (int)(col / M);
         float wsum = 0.f;
-        for (int t = 0; t < ntiles; ++t) wsum = __fadd_rn(wsum, tile_w(tiles[t], m, n));
+        for (int t = 0; t < ntiles; ++t) wsum = wsum + tile_w(tiles[t], m, n);
         for (int p = 0; p < P; ++p) out[col + ncol * p] = 0.f;
         for (int t = 0; t < ntiles; ++t) {
             const TileDev& T = tiles[t];
             if (m < T.xpos[0] || m > T.xpos[1] || n < T.xpos[2] || n > T.xpos[3]) continue;
-            const float w = __fdiv_rn(tile_w(T, m, n), wsum);
+            const float w = tile_w(T, m, n) / wsum;
             const size_t base = (size_t)(m - T.xpos[0]) + (size_t)T.xsize[0] * (n - T.xpos[2]);
             const size_t plane = (size_t)T.xsize[0] * T.xsize[1];
             for (int p = T.xpos[4]; p <= T.xpos[5]; ++p) {
@@ -305,9 +309,7 @@ __global__ __launch_bounds__(STHR) void k_tile_blend(const TileDev* __restrict__
                     s += -offset;
                     s = s < 0 ? 0.f : s;
                 }
-                // product and sum rounded separately, as the reference's
-                // buf[pos] += w*S (no fused multiply-add)
-                out[col + ncol * p] = __fadd_rn(out[col + ncol * p], __fmul_rn(w, s));
+                out[col + ncol * p] = out[col + ncol * p] + w * s;
             }
         }
     }

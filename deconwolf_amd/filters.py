@@ -71,4 +71,4 @@ def gsmooth(V, sigma):
         while Ka.shape[0] > 2 * P - 1:
             Ka = Ka[1:-1]
         out = conv1_normalized(out, Ka, 0)
-    return out
+    return np.ascontiguousarray(out)

@@ -148,6 +148,7 @@ def _run_host(entry, im, psf, s):
     p = _params(s)
     start = _lp_start(im, s)
     if start is not None:
+        start = np.ascontiguousarray(start, dtype=np.float32)  # the library reads it x-fastest
         p.start_image = start.ctypes.data
     out = np.empty_like(im)
     nerr = max(int(s.nIter), int(s.maxiter), 1)

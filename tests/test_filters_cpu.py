@@ -48,6 +48,8 @@ def test_gsmooth_constant_and_trim():
     V = np.full((3, 20, 30), 7.0, np.float32)
     out = filters.gsmooth(V, 8.0)
     np.testing.assert_allclose(out, 7.0, rtol=1e-6)   # normalised edges keep constants
+    # handed to the library as an x-fastest buffer
+    assert out.flags["C_CONTIGUOUS"] and out.dtype == np.float32
 
 
 def test_bw_in_focus_is_airy():
