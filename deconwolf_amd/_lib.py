@@ -141,13 +141,15 @@ _lib = None
 
 
 def lib():
-    """Loads libdwhip.so; raises if it has not been built."""
+    """Loads libdwhip.so (or the build named by $DWHIP_LIBRARY, e.g. one
+    made with other tile parameters); raises if it has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = os.environ.get("DWHIP_LIBRARY") or LIB_PATH
+        if not os.path.exists(path):
             raise RuntimeError(
-                "libdwhip.so is not built (expected at %s); run `python -m deconwolf_amd.build`" % LIB_PATH)
-        L = ctypes.CDLL(LIB_PATH)
+                "libdwhip.so is not built (expected at %s); run `python -m deconwolf_amd.build`" % path)
+        L = ctypes.CDLL(path)
         for name, res, args in SIGNATURES:
             f = getattr(L, name)
             f.restype = res

@@ -1,6 +1,10 @@
 """Builds libdwhip.so in-tree with hipcc for gfx950.
 
-    python -m deconwolf_amd.build [-v] [-j N]
+    python -m deconwolf_amd.build [-v] [-j N] [-D NAME=VALUE ...] [-o LIB]
+
+-D passes a define to every unit (the tile parameters DWHIP_XTHR,
+DWHIP_CTHR, DWHIP_ZTHR, DWHIP_XMAXE, DWHIP_CMAXE); -o builds that
+variant elsewhere (objects in a folder beside it).
 
 Every pass-kernel mode is its own translation unit (k_xpass.hip and
 k_colpass.hip compiled once per mode), compiled in parallel, then linked.
@@ -55,10 +59,10 @@ def _filter(stderr):
     return "\n".join(l for l in stderr.splitlines() if "packed-fp32-ops' is not a recognized" not in l)
 
 
-def _compile(unit, verbose):
+def _compile(unit, verbose, extra=(), objdir=OBJ):
     src, defs, obj = unit
-    cmd = [hipcc()] + FLAGS + defs + ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-c",
-                                     os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
+    cmd = [hipcc()] + FLAGS + list(extra) + defs + ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-c",
+                                                     os.path.join(CSRC, src), "-o", os.path.join(objdir, obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msg = _filter(r.stderr)
     if r.returncode != 0:
@@ -68,24 +72,28 @@ def _compile(unit, verbose):
     return obj
 
 
-def build(verbose=False, jobs=None):
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose=False, jobs=None, defines=(), lib=None):
+    lib = lib or LIB
+    objdir = OBJ if lib == LIB else lib + ".objs"
+    os.makedirs(objdir, exist_ok=True)
+    extra = ["-D" + d for d in defines]
     us = units()
     jobs = jobs or min(len(us), os.cpu_count() or 4, 16)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda u: _compile(u, verbose), us))
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + \
-          [os.path.join(OBJ, o) for o in objs]
+        objs = list(ex.map(lambda u: _compile(u, verbose, extra, objdir), us))
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", lib + ".tmp"] + \
+          [os.path.join(objdir, o) for o in objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + _filter(r.stderr))
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(lib + ".tmp", lib)
+    return lib
 
 
 if __name__ == "__main__":
     j = None
     if "-j" in sys.argv:
         j = int(sys.argv[sys.argv.index("-j") + 1])
-    build(verbose="-v" in sys.argv, jobs=j)
-    print(LIB)
+    defs = [sys.argv[i + 1] for i, a in enumerate(sys.argv) if a == "-D"]
+    out = sys.argv[sys.argv.index("-o") + 1] if "-o" in sys.argv else None
+    print(build(verbose="-v" in sys.argv, jobs=j, defines=defs, lib=out))

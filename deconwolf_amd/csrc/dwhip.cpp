@@ -261,7 +261,7 @@ static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 size_t dwhip_job_bytes(const int64_t* work, int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN,
                        int64_t pP, int border_quality, int niter)
 {
-    const size_t h = work[0] / 2, hcs = round_up(h + 1, 8);
+    const size_t h = work[0] / 2, hcs = round_up(h + 1, 16);
     const size_t V = (size_t)work[0] * work[1] * work[2];
     const size_t Vc = hcs * work[1] * work[2];
     DwhipTiles t;
@@ -329,7 +329,7 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.nz = (int)w[2];
     d.h = d.nx / 2;
     d.hc = d.h + 1;
-    d.hcs = (int)round_up(d.hc, 8);
+    d.hcs = (int)round_up(d.hc, 16);  // rows start on 128-byte lines
     d.M = (int)M; d.N = (int)N; d.P = (int)P;
     d.pM = (int)pM; d.pN = (int)pN; d.pP = (int)pP;
     d.refMNP = (double)ref[0] * (double)ref[1] * (double)ref[2];
