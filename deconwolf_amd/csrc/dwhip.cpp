@@ -129,6 +129,8 @@ static bool size_ok(const int64_t* w)
     // DWHIP_CMAXE*NTHR per column tile, 2^20 for the index arithmetic
     if (w[0] / 2 > DWHIP_XMAXE * DWHIP_XTHR) return false;
     if (w[1] > DWHIP_CMAXE * DWHIP_CTHR || w[2] > DWHIP_CMAXE * DWHIP_ZTHR) return false;
+    // and one pencil per column tile must fit in LDS
+    if (std::max(w[1], w[2]) * (int64_t)sizeof(float2) > DWHIP_MAX_LDS_BYTES) return false;
     if (w[0] < 2 || w[1] < 1 || w[2] < 1) return false;
     if (w[0] / 2 > 1 && dwhip_radices((int)(w[0] / 2), true).empty()) return false;
     if (w[1] > 1 && dwhip_radices((int)w[1], false).empty()) return false;
@@ -246,8 +248,11 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
     const int h = (int)(w[0] / 2), ny = (int)w[1], nz = (int)w[2];
     const int hc = h + 1;
     t->xrows = std::max(1, std::min(64, DWHIP_XMAXE * DWHIP_XTHR / h));
-    t->ycols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_CTHR / ny)));
-    t->zcols = std::max(1, std::min(64, pow2_floor(DWHIP_CMAXE * DWHIP_ZTHR / nz)));
+    // a column tile holds B pencils of n complex values in LDS
+    const int ylim = std::min(DWHIP_CMAXE * DWHIP_CTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
+    const int zlim = std::min(DWHIP_CMAXE * DWHIP_ZTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
+    t->ycols = std::max(1, std::min(64, pow2_floor(ylim / ny)));
+    t->zcols = std::max(1, std::min(64, pow2_floor(zlim / nz)));
     t->ntile[KIND_X] = (ny * nz + t->xrows - 1) / t->xrows;
     t->ntile[KIND_Y] = ((hc + t->ycols - 1) / t->ycols) * nz;
     t->ntile[KIND_Z] = ((hc + t->zcols - 1) / t->zcols) * ny;

@@ -24,11 +24,14 @@ __host__ __device__ constexpr bool cmode_is_y(int m)
 #define DWHIP_XTHR 256   // threads per X-pass workgroup
 #endif
 #ifndef DWHIP_CTHR
-#define DWHIP_CTHR 512   // threads per workgroup, Y passes (long pencils)
+#define DWHIP_CTHR 1024  // threads per workgroup, Y passes (long pencils)
 #endif
 #ifndef DWHIP_ZTHR
 #define DWHIP_ZTHR 256   // threads per workgroup, Z passes
 #endif
+// dynamic LDS a workgroup may use: gfx950's 160 KiB less 1 KiB kept for
+// static LDS (the limit the kernels are raised to)
+#define DWHIP_MAX_LDS_BYTES (159 * 1024)
 #ifndef DWHIP_XMAXE
 #define DWHIP_XMAXE 25   // complex elements per thread per stage, X pass
 #endif

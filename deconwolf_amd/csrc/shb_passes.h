@@ -377,7 +377,7 @@ constexpr int col_threads()
     return cmode_is_y(MODE) ? CTHR : ZTHR;
 }
 
-// Y tiles (512 threads, up to 80 KiB of LDS) fit two workgroups per CU
+// Y tiles (1024 threads, up to 159 KiB of LDS) need 4 waves per SIMD
 // when the kernel stays within 128 VGPRs.
 template <int MODE>
 __global__ __launch_bounds__(col_threads<MODE>()) __attribute__((amdgpu_waves_per_eu(cmode_is_y(MODE) ? 4 : 1)))
