@@ -3,7 +3,7 @@
     python -m deconwolf_amd.build [-v] [-j N] [-D NAME=VALUE ...] [-o LIB]
 
 -D passes a define to every unit (the tile parameters DWHIP_XTHR,
-DWHIP_CTHR, DWHIP_ZTHR, DWHIP_XMAXE, DWHIP_CMAXE); -o builds that
+DWHIP_CTHR, DWHIP_ZTHR, DWHIP_XMAXE, DWHIP_CMAXE, DWHIP_YMAXE, DWHIP_ZMAXE); -o builds that
 variant elsewhere (objects in a folder beside it).
 
 Every pass-kernel mode is its own translation unit (k_xpass.hip and

@@ -86,12 +86,12 @@ static bool is_smooth(int64_t n)
     return n == 1;
 }
 
-std::vector<int> dwhip_radices(int n, bool rows);
+std::vector<int> dwhip_radices(int n, int axis);
 
-// smallest length >= n with a column plan
-static int64_t next_smooth(int64_t n)
+// smallest length >= n with a plan along axis 1 (y) or 2 (z)
+static int64_t next_smooth(int64_t n, int axis)
 {
-    while (!(is_smooth(n) && (n == 1 || !dwhip_radices((int)n, false).empty()))) ++n;
+    while (!(is_smooth(n) && (n == 1 || !dwhip_radices((int)n, axis).empty()))) ++n;
     return n;
 }
 
@@ -99,7 +99,7 @@ static int64_t next_smooth(int64_t n)
 static int64_t next_smooth_even(int64_t n)
 {
     int64_t h = (n + 1) / 2;
-    while (!(is_smooth(h) && (h == 1 || !dwhip_radices((int)h, true).empty()))) ++h;
+    while (!(is_smooth(h) && (h == 1 || !dwhip_radices((int)h, 0).empty()))) ++h;
     return 2 * h;
 }
 
@@ -126,15 +126,15 @@ static bool size_ok(const int64_t* w)
 {
     if (w[0] % 2 != 0 || !is_smooth(w[0] / 2) || !is_smooth(w[1]) || !is_smooth(w[2])) return false;
     // kernel limits: at most DWHIP_XMAXE*NTHR complex per x-row tile,
-    // DWHIP_CMAXE*NTHR per column tile, 2^20 for the index arithmetic
+    // DWHIP_YMAXE / DWHIP_ZMAXE * NTHR per column tile
     if (w[0] / 2 > DWHIP_XMAXE * DWHIP_XTHR) return false;
-    if (w[1] > DWHIP_CMAXE * DWHIP_CTHR || w[2] > DWHIP_CMAXE * DWHIP_ZTHR) return false;
+    if (w[1] > DWHIP_YMAXE * DWHIP_CTHR || w[2] > DWHIP_ZMAXE * DWHIP_ZTHR) return false;
     // and one pencil per column tile must fit in LDS
     if (std::max(w[1], w[2]) * (int64_t)sizeof(float2) > DWHIP_MAX_LDS_BYTES) return false;
     if (w[0] < 2 || w[1] < 1 || w[2] < 1) return false;
-    if (w[0] / 2 > 1 && dwhip_radices((int)(w[0] / 2), true).empty()) return false;
-    if (w[1] > 1 && dwhip_radices((int)w[1], false).empty()) return false;
-    if (w[2] > 1 && dwhip_radices((int)w[2], false).empty()) return false;
+    if (w[0] / 2 > 1 && dwhip_radices((int)(w[0] / 2), 0).empty()) return false;
+    if (w[1] > 1 && dwhip_radices((int)w[1], 1).empty()) return false;
+    if (w[2] > 1 && dwhip_radices((int)w[2], 2).empty()) return false;
     return true;
 }
 
@@ -150,8 +150,8 @@ int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int
     int64_t w[3];
     if (border_quality == 2) {
         w[0] = next_smooth_even(r[0]);
-        w[1] = next_smooth(r[1]);
-        w[2] = next_smooth(r[2]);
+        w[1] = next_smooth(r[1], 1);
+        w[2] = next_smooth(r[2], 2);
     } else {
         for (int d = 0; d < 3; ++d) w[d] = r[d];
     }
@@ -163,10 +163,13 @@ int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int
 // Radices for a 2,3,5,7-smooth n: fewest stages, then the smallest largest
 // radix (fewer registers per butterfly), from the radices the kernels
 // instantiate (DWHIP_FOR_RADICES in fft_engine.h).
-static const int kRadicesCols[] = {16, 15, 9, 8, 7, 6, 5, 4, 3, 2};
-static const int kRadicesRows[] = {25, 20, 16, 15, 9, 8, 7, 6, 5, 4, 3, 2};
+static const int kRadices[] = {25, 20, 16, 15, 9, 8, 7, 6, 5, 4, 3, 2};
+static int axis_rmax(int axis)
+{
+    return axis == 0 ? DWHIP_RMAX_X : axis == 1 ? DWHIP_RMAX_Y : DWHIP_RMAX_Z;
+}
 
-static void radix_search(int n, bool rows, std::vector<int>& cur, std::vector<int>& best)
+static void radix_search(int n, int rmax, std::vector<int>& cur, std::vector<int>& best)
 {
     if (n == 1) {
         if (best.empty() || cur.size() < best.size() ||
@@ -177,25 +180,25 @@ static void radix_search(int n, bool rows, std::vector<int>& cur, std::vector<in
     }
     if (cur.size() >= DWHIP_MAX_FFT_STAGES) return;
     if (!best.empty() && cur.size() + 1 > best.size()) return;
-    const int* rs = rows ? kRadicesRows : kRadicesCols;
-    const int nr = rows ? (int)(sizeof(kRadicesRows) / sizeof(int)) : (int)(sizeof(kRadicesCols) / sizeof(int));
+    const int nr = (int)(sizeof(kRadices) / sizeof(int));
     for (int i = 0; i < nr; ++i) {
-        const int r = rs[i];
+        const int r = kRadices[i];
+        if (r > rmax) continue;
         // non-increasing radices: each factorisation is visited once
         if (!cur.empty() && r > cur.back()) continue;
         if (n % r != 0) continue;
         cur.push_back(r);
-        radix_search(n / r, rows, cur, best);
+        radix_search(n / r, rmax, cur, best);
         cur.pop_back();
     }
 }
 
-// Radices for a 2,3,5,7-smooth n (rows: x-row transforms), at most
-// DWHIP_MAX_FFT_STAGES of them; empty when n has no such plan.
-std::vector<int> dwhip_radices(int n, bool rows)
+// Radices for a 2,3,5,7-smooth n along axis 0 (x rows), 1 (y) or 2 (z), at
+// most DWHIP_MAX_FFT_STAGES of them; empty when n has no such plan.
+std::vector<int> dwhip_radices(int n, int axis)
 {
     std::vector<int> cur, best;
-    if (n > 1) radix_search(n, rows, cur, best);
+    if (n > 1) radix_search(n, axis_rmax(axis), cur, best);
     return best;
 }
 
@@ -225,9 +228,9 @@ int dwhip_make_plan(int n, const std::vector<int>& r, FftPlanDev* p, std::vector
 }
 
 // forward (radices R1..Rk) and inverse (Rk..R1) plans of length n
-static int make_plan_pair(int n, bool rows, FftPlanDev* f, FftPlanDev* i, std::vector<float2>& tw)
+static int make_plan_pair(int n, int axis, FftPlanDev* f, FftPlanDev* i, std::vector<float2>& tw)
 {
-    std::vector<int> r = dwhip_radices(n, rows);
+    std::vector<int> r = dwhip_radices(n, axis);
     if (n > 1 && r.empty()) return DWHIP_ERR_SIZE;
     std::vector<int> rr(r.rbegin(), r.rend());
     int rc = dwhip_make_plan(n, r, f, tw);
@@ -249,8 +252,8 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
     const int hc = h + 1;
     t->xrows = std::max(1, std::min(64, DWHIP_XMAXE * DWHIP_XTHR / h));
     // a column tile holds B pencils of n complex values in LDS
-    const int ylim = std::min(DWHIP_CMAXE * DWHIP_CTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
-    const int zlim = std::min(DWHIP_CMAXE * DWHIP_ZTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
+    const int ylim = std::min(DWHIP_YMAXE * DWHIP_CTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
+    const int zlim = std::min(DWHIP_ZMAXE * DWHIP_ZTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
     t->ycols = std::max(1, std::min(64, pow2_floor(ylim / ny)));
     t->zcols = std::max(1, std::min(64, pow2_floor(zlim / nz)));
     t->ntile[KIND_X] = (ny * nz + t->xrows - 1) / t->xrows;
@@ -357,8 +360,8 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.start_img = start_dev;
 
     std::vector<float2>& tw = j->tw;
-    if (make_plan_pair(d.h, true, &d.pxf, &d.pxi, tw) || make_plan_pair(d.ny, false, &d.pyf, &d.pyi, tw) ||
-        make_plan_pair(d.nz, false, &d.pzf, &d.pzi, tw))
+    if (make_plan_pair(d.h, 0, &d.pxf, &d.pxi, tw) || make_plan_pair(d.ny, 1, &d.pyf, &d.pyi, tw) ||
+        make_plan_pair(d.nz, 2, &d.pzf, &d.pzi, tw))
         return DWHIP_ERR_SIZE;
     j->rtw_off = tw.size();
     for (int k = 0; k < d.h; ++k) {

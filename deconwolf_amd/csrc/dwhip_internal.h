@@ -53,7 +53,7 @@ void dwhip_prof_end(hipStream_t s, int slot, int kernel, double bytes);
 void dwhip_prof_collect();
 double dwhip_pass_bytes(int xkind, int mode, const JobDev& d, int iter);
 int dwhip_make_plan(int n, const std::vector<int>& radices, FftPlanDev* p, std::vector<float2>& tw);
-std::vector<int> dwhip_radices(int n, bool rows);
+std::vector<int> dwhip_radices(int n, int axis);
 void dwhip_tile_config(const int64_t* w, DwhipTiles* t);
 int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64_t P, const float* psf_dev,
                    int64_t pM, int64_t pN, int64_t pP, const dwhip_params* prm, const float* start_dev);

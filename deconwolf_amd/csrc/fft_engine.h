@@ -277,6 +277,36 @@ template <int R, int DIR> struct DftC<R, DIR, false> {
 };
 
 // ---------------------------------------------------------------------------
+// global-memory pointers
+// ---------------------------------------------------------------------------
+// Pointers read from the job table are generic (flat) to the compiler, and
+// flat loads also count against the LDS wait counter, so every LDS wait
+// would wait for outstanding memory loads too.  Casting them to the global
+// address space gives global loads with a scalar base.
+#define DWHIP_GL __attribute__((address_space(1)))
+typedef const DWHIP_GL float2* TwPtr;
+template <class T>
+__device__ __forceinline__ DWHIP_GL T* gp(T* p)
+{
+    return (DWHIP_GL T*)p;
+}
+// float2's constructors take generic references, so complex values move
+// through global pointers as native two-float vectors
+typedef float dwhip_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ldg(const DWHIP_GL float2* p)
+{
+    const dwhip_f2v t = *(const DWHIP_GL dwhip_f2v*)p;
+    return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ void stg(DWHIP_GL float2* p, float2 v)
+{
+    dwhip_f2v t;
+    t.x = v.x;
+    t.y = v.y;
+    *(DWHIP_GL dwhip_f2v*)p = t;
+}
+
+// ---------------------------------------------------------------------------
 // stage geometry
 // ---------------------------------------------------------------------------
 struct StageGeo {
@@ -307,12 +337,12 @@ __device__ __forceinline__ void bfly_of(const StageGeo& g, int w, int nb, uint32
 }
 
 template <int R, int DIR>
-__device__ __forceinline__ void twiddle(float2* v, const float2* __restrict__ tw, int k)
+__device__ __forceinline__ void twiddle(float2* v, TwPtr __restrict__ tw, int k)
 {
     if (k == 0) return;
-    const float2* t = tw + k * (R - 1);
+    TwPtr t = tw + k * (R - 1);
 #pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = DIR < 0 ? cmul(v[r], t[r - 1]) : cmulc(v[r], t[r - 1]);
+    for (int r = 1; r < R; ++r) v[r] = DIR < 0 ? cmul(v[r], ldg(t + r - 1)) : cmulc(v[r], ldg(t + r - 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -358,7 +388,7 @@ __device__ __forceinline__ void stage_first(const StageGeo& g, int tid, Src src,
 // middle stage: LDS -> twiddle -> DFT -> LDS (in place, two barriers)
 template <int R, int MB, int DIR, bool COLS>
 __device__ __forceinline__ void stage_mid(const StageGeo& g, int tid, float2* lds, int ns, uint32_t nsmagic,
-                                          const float2* __restrict__ tw)
+                                          TwPtr __restrict__ tw)
 {
     const int nb = g.n / R;
     const uint32_t nbm = fdiv_magic((uint32_t)nb);
@@ -396,7 +426,7 @@ __device__ __forceinline__ void stage_mid(const StageGeo& g, int tid, float2* ld
 // last stage (ns = n/R, so the outputs of butterfly j are j + r*nb):
 // LDS -> twiddle -> DFT -> Dst
 template <int R, int MB, int DIR, bool COLS, bool SYNC, class Dst>
-__device__ __forceinline__ void stage_last(const StageGeo& g, int tid, const float2* lds, const float2* __restrict__ tw,
+__device__ __forceinline__ void stage_last(const StageGeo& g, int tid, const float2* lds, TwPtr __restrict__ tw,
                                            Dst dst)
 {
     const int nb = g.n / R;
@@ -432,9 +462,9 @@ __device__ __forceinline__ void stage_last(const StageGeo& g, int tid, const flo
 // (last stage of a DIR1 transform met by the first stage of a DIR2 one).
 // Op: float2 fetch(int pen, int i) const        operand of element i
 //     float2 apply(float2 v, float2 f) const     (element j + r*nb)
-template <int R, int MB, int DIR1, int DIR2, bool COLS, class Op>
-__device__ __forceinline__ void stage_junction(const StageGeo& g, int tid, float2* lds, const float2* __restrict__ tw,
-                                               Op op)
+template <int R, int MB, int DIR1, int DIR2, bool COLS, bool PF, class Op>
+__device__ __forceinline__ void stage_junction(const StageGeo& g, int tid, float2* lds, TwPtr __restrict__ tw,
+                                               Op op, const float2* pf)
 {
     const int nb = g.n / R;
     const uint32_t nbm = fdiv_magic((uint32_t)nb);
@@ -460,13 +490,35 @@ __device__ __forceinline__ void stage_junction(const StageGeo& g, int tid, float
             twiddle<R, DIR1>(v[q], tw, j);
             DftC<R, DIR1>::run(v[q]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = op.apply(v[q][r], op.fetch(pen, j + r * nb));
+            for (int r = 0; r < R; ++r)
+                v[q][r] = op.apply(v[q][r], PF ? pf[q * R + r] : op.fetch(pen, j + r * nb));
             DftC<R, DIR2>::run(v[q]);
 #pragma unroll
             for (int r = 0; r < R; ++r) lds[lidx<COLS>(g, pen, j * R + r)] = v[q][r];
         }
     }
     __syncthreads();
+}
+
+// Operand prefetch for a junction: the values op.fetch returns there, for
+// the same butterflies and in the same order, loaded before the first stage
+// of the pair so that their latency overlaps the forward stages.
+template <int R, int MB, bool COLS, class Op>
+__device__ __forceinline__ void stage_prefetch(const StageGeo& g, int tid, const Op& op, float2* pf)
+{
+    const int nb = g.n / R;
+    const uint32_t nbm = fdiv_magic((uint32_t)nb);
+    const int total = nb * g.npen;
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+        const int w = tid + q * g.nthr;
+        if (w < total) {
+            int pen, j;
+            bfly_of<COLS>(g, w, nb, nbm, pen, j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) pf[q * R + r] = op.fetch(pen, j + r * nb);
+        }
+    }
 }
 
 // single-stage transform pair (n == R): Src -> DFT(DIR1) -> op -> DFT(DIR2)
@@ -511,73 +563,73 @@ struct NoOp {
 // radix dispatch
 // ---------------------------------------------------------------------------
 // Radices the planner may use; each maps to a template instantiation.
-// Column tiles (COLS) use radices up to 16; x-rows also 20 and 25 (their
-// tiles hold fewer elements per thread).
+// Each axis instantiates the radices up to its own limit RMAX: x-rows up
+// to 25, y columns up to 16, z columns up to 7 (the z kernels hold the
+// OTF operand of the junction in registers too, and the smaller radices
+// keep them within 128 VGPRs).
 #define DWHIP_FOR_RADICES(X) \
     X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(15) X(16) X(20) X(25)
-__host__ __device__ constexpr bool radix_ok(bool cols, int R) { return !cols || R <= 16; }
+#define DWHIP_RMAX_X 25
+#define DWHIP_RMAX_Y 16
+#define DWHIP_RMAX_Z 7
+__host__ __device__ constexpr bool radix_ok(int rmax, int R) { return R <= rmax; }
 #define DWHIP_MAX_FFT_STAGES 4
 
 __host__ __device__ constexpr int mb_for(int maxe, int R) { return (maxe + R - 1) / R; }
 
-template <int MAXE, int DIR, bool COLS, bool SYNC, class Src>
+template <int MAXE, int DIR, bool COLS, int RMAX, bool SYNC, class Src>
 __device__ __forceinline__ void run_first(int R, const StageGeo& g, int tid, Src src, float2* lds)
 {
     auto dst = [&](int pen, int i, float2 v) { lds[lidx<COLS>(g, pen, i)] = v; };
     switch (R) {
 #define CASE(RR) \
-    case RR: if constexpr (radix_ok(COLS, RR)) { stage_first<RR, mb_for(MAXE, RR), DIR, COLS, SYNC>(g, tid, src, dst); } break;
+    case RR: if constexpr (radix_ok(RMAX, RR)) { stage_first<RR, mb_for(MAXE, RR), DIR, COLS, SYNC>(g, tid, src, dst); } break;
         DWHIP_FOR_RADICES(CASE)
 #undef CASE
     default: break;
     }
 }
 
-template <int MAXE, int DIR, bool COLS>
+template <int MAXE, int DIR, bool COLS, int RMAX>
 __device__ __forceinline__ void run_mid(int R, const StageGeo& g, int tid, float2* lds, int ns, uint32_t nsmagic,
-                                        const float2* __restrict__ tw)
+                                        TwPtr __restrict__ tw)
 {
     switch (R) {
 #define CASE(RR) \
-    case RR: if constexpr (radix_ok(COLS, RR)) { stage_mid<RR, mb_for(MAXE, RR), DIR, COLS>(g, tid, lds, ns, nsmagic, tw); } break;
+    case RR: if constexpr (radix_ok(RMAX, RR)) { stage_mid<RR, mb_for(MAXE, RR), DIR, COLS>(g, tid, lds, ns, nsmagic, tw); } break;
         DWHIP_FOR_RADICES(CASE)
 #undef CASE
     default: break;
     }
 }
 
-template <int MAXE, int DIR, bool COLS, bool SYNC, class Dst>
+template <int MAXE, int DIR, bool COLS, int RMAX, bool SYNC, class Dst>
 __device__ __forceinline__ void run_last(int R, const StageGeo& g, int tid, const float2* lds,
-                                         const float2* __restrict__ tw, Dst dst)
+                                         TwPtr __restrict__ tw, Dst dst)
 {
     switch (R) {
 #define CASE(RR) \
-    case RR: if constexpr (radix_ok(COLS, RR)) { stage_last<RR, mb_for(MAXE, RR), DIR, COLS, SYNC>(g, tid, lds, tw, dst); } break;
+    case RR: if constexpr (radix_ok(RMAX, RR)) { stage_last<RR, mb_for(MAXE, RR), DIR, COLS, SYNC>(g, tid, lds, tw, dst); } break;
         DWHIP_FOR_RADICES(CASE)
 #undef CASE
     default: break;
     }
 }
 
-template <int MAXE, int DIR1, int DIR2, bool COLS, class Op>
-__device__ __forceinline__ void run_junction(int R, const StageGeo& g, int tid, float2* lds,
-                                             const float2* __restrict__ tw, Op op)
-{
-    switch (R) {
-#define CASE(RR) \
-    case RR: if constexpr (radix_ok(COLS, RR)) { stage_junction<RR, mb_for(MAXE, RR), DIR1, DIR2, COLS>(g, tid, lds, tw, op); } break;
-        DWHIP_FOR_RADICES(CASE)
-#undef CASE
-    default: break;
-    }
-}
+// Junction operands held in registers between the prefetch and the
+// junction: at most DWHIP_PF_MAX per thread; junction radices needing more
+// fetch their operands at the junction.
+#ifndef DWHIP_PF_MAX
+#define DWHIP_PF_MAX 20
+#endif
+__host__ __device__ constexpr bool pf_ok(int maxe, int R) { return mb_for(maxe, R) * R <= DWHIP_PF_MAX; }
 
-template <int MAXE, int DIR1, int DIR2, bool COLS, bool PAIR, bool SYNC, class Src, class Op, class Dst>
+template <int MAXE, int DIR1, int DIR2, bool COLS, int RMAX, bool PAIR, bool SYNC, class Src, class Op, class Dst>
 __device__ __forceinline__ void run_single(int R, const StageGeo& g, int tid, Src src, Op op, Dst dst)
 {
     switch (R) {
 #define CASE(RR) \
-    case RR: if constexpr (radix_ok(COLS, RR)) { stage_single<RR, mb_for(MAXE, RR), DIR1, DIR2, COLS, PAIR, SYNC>(g, tid, src, op, dst); } break;
+    case RR: if constexpr (radix_ok(RMAX, RR)) { stage_single<RR, mb_for(MAXE, RR), DIR1, DIR2, COLS, PAIR, SYNC>(g, tid, src, op, dst); } break;
         DWHIP_FOR_RADICES(CASE)
 #undef CASE
     default: break;
@@ -588,24 +640,24 @@ __device__ __forceinline__ void run_single(int R, const StageGeo& g, int tid, Sr
 // whole transforms on a tile
 // ---------------------------------------------------------------------------
 // Stages 1..nst-2 of plan p (all but the first and the last); nst <= 4.
-template <int MAXE, int DIR, bool COLS>
+template <int MAXE, int DIR, bool COLS, int RMAX>
 __device__ __forceinline__ void run_mids(const FftPlanDev& p, const StageGeo& g, int tid, float2* lds,
-                                         const float2* __restrict__ tw)
+                                         TwPtr __restrict__ tw)
 {
     // At most four stages: straight-line code, so that the register
     // allocator sees disjoint stage bodies (a loop around the radix switch
     // makes their live ranges overlap).
-    if (p.nst > 2) run_mid<MAXE, DIR, COLS>(p.radix[1], g, tid, lds, p.ns[1], p.nsmagic[1], tw + p.twoff[1]);
-    if (p.nst > 3) run_mid<MAXE, DIR, COLS>(p.radix[2], g, tid, lds, p.ns[2], p.nsmagic[2], tw + p.twoff[2]);
+    if (p.nst > 2) run_mid<MAXE, DIR, COLS, RMAX>(p.radix[1], g, tid, lds, p.ns[1], p.nsmagic[1], tw + p.twoff[1]);
+    if (p.nst > 3) run_mid<MAXE, DIR, COLS, RMAX>(p.radix[2], g, tid, lds, p.ns[2], p.nsmagic[2], tw + p.twoff[2]);
 }
 
 // Src -> transform -> Dst.  The LDS must be free on entry.  SRC_LDS: Src
 // reads the LDS (a barrier then separates its reads from the first
 // stage's writes); DST_LDS: Dst writes the LDS (likewise for the last
 // stage).  On return the caller synchronises before reusing the LDS.
-template <int MAXE, int DIR, bool COLS, bool SRC_LDS, bool DST_LDS, class Src, class Dst>
+template <int MAXE, int DIR, bool COLS, int RMAX, bool SRC_LDS, bool DST_LDS, class Src, class Dst>
 __device__ __forceinline__ void fft_tile(const FftPlanDev& p, const StageGeo& g, int tid, float2* lds,
-                                         const float2* __restrict__ tw, Src src, Dst dst)
+                                         TwPtr __restrict__ tw, Src src, Dst dst)
 {
     if (p.nst == 0) {  // n == 1
         float2 v[MAXE];
@@ -619,33 +671,50 @@ __device__ __forceinline__ void fft_tile(const FftPlanDev& p, const StageGeo& g,
         return;
     }
     if (p.nst == 1) {
-        run_single<MAXE, DIR, 0, COLS, false, SRC_LDS || DST_LDS>(p.radix[0], g, tid, src, NoOp(), dst);
+        run_single<MAXE, DIR, 0, COLS, RMAX, false, SRC_LDS || DST_LDS>(p.radix[0], g, tid, src, NoOp(), dst);
         return;
     }
-    run_first<MAXE, DIR, COLS, SRC_LDS>(p.radix[0], g, tid, src, lds);
+    run_first<MAXE, DIR, COLS, RMAX, SRC_LDS>(p.radix[0], g, tid, src, lds);
     __syncthreads();
-    run_mids<MAXE, DIR, COLS>(p, g, tid, lds, tw);
-    run_last<MAXE, DIR, COLS, DST_LDS>(p.radix[p.nst - 1], g, tid, lds, tw + p.twoff[p.nst - 1], dst);
+    run_mids<MAXE, DIR, COLS, RMAX>(p, g, tid, lds, tw);
+    run_last<MAXE, DIR, COLS, RMAX, DST_LDS>(p.radix[p.nst - 1], g, tid, lds, tw + p.twoff[p.nst - 1], dst);
 }
 
 // Src -> transform (plan a, DIR1) -> op -> transform (plan b, DIR2) -> Dst,
 // b's first radix equal to a's last.  Src and Dst address global memory.
-template <int MAXE, int DIR1, int DIR2, bool COLS, class Src, class Op, class Dst>
+template <int MAXE, int DIR1, int DIR2, bool COLS, int RMAX, class Src, class Op, class Dst>
 __device__ __forceinline__ void fft_pair_tile(const FftPlanDev& a, const FftPlanDev& b, const StageGeo& g, int tid,
-                                              float2* lds, const float2* __restrict__ tw, Src src, Op op, Dst dst)
+                                              float2* lds, TwPtr __restrict__ tw, Src src, Op op, Dst dst)
 {
     if (a.nst == 0) {  // n == 1: the op alone
         for (int w = tid; w < g.npen; w += g.nthr) dst(w, 0, op.apply(src(w, 0), op.fetch(w, 0)));
         return;
     }
     if (a.nst == 1) {
-        run_single<MAXE, DIR1, DIR2, COLS, true, false>(a.radix[0], g, tid, src, op, dst);
+        run_single<MAXE, DIR1, DIR2, COLS, RMAX, true, false>(a.radix[0], g, tid, src, op, dst);
         return;
     }
-    run_first<MAXE, DIR1, COLS, false>(a.radix[0], g, tid, src, lds);
-    __syncthreads();
-    run_mids<MAXE, DIR1, COLS>(a, g, tid, lds, tw);
-    run_junction<MAXE, DIR1, DIR2, COLS>(a.radix[a.nst - 1], g, tid, lds, tw + a.twoff[a.nst - 1], op);
-    run_mids<MAXE, DIR2, COLS>(b, g, tid, lds, tw);
-    run_last<MAXE, DIR2, COLS, false>(b.radix[b.nst - 1], g, tid, lds, tw + b.twoff[b.nst - 1], dst);
+    // One code path per junction radix, so that the prefetched operands
+    // live in registers of a statically known count across the forward
+    // stages.
+    switch (a.radix[a.nst - 1]) {
+#define CASE(RR)                                                                                           \
+    case RR:                                                                                               \
+        if constexpr (radix_ok(RMAX, RR)) {                                                                \
+            constexpr int MB = mb_for(MAXE, RR);                                                           \
+            constexpr bool PF = pf_ok(MAXE, RR);                                                           \
+            float2 pf[PF ? MB * RR : 1];                                                                   \
+            if (PF) stage_prefetch<RR, MB, COLS>(g, tid, op, pf);                                          \
+            run_first<MAXE, DIR1, COLS, RMAX, false>(a.radix[0], g, tid, src, lds);                        \
+            __syncthreads();                                                                               \
+            run_mids<MAXE, DIR1, COLS, RMAX>(a, g, tid, lds, tw);                                          \
+            stage_junction<RR, MB, DIR1, DIR2, COLS, PF>(g, tid, lds, tw + a.twoff[a.nst - 1], op, pf);    \
+            run_mids<MAXE, DIR2, COLS, RMAX>(b, g, tid, lds, tw);                                          \
+            run_last<MAXE, DIR2, COLS, RMAX, false>(b.radix[b.nst - 1], g, tid, lds, tw + b.twoff[b.nst - 1], dst); \
+        }                                                                                                  \
+        break;
+        DWHIP_FOR_RADICES(CASE)
+#undef CASE
+    default: break;
+    }
 }

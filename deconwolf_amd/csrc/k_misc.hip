@@ -130,8 +130,19 @@ __global__ __launch_bounds__(STHR) void k_round_end(const BatchDev* __restrict__
     __shared__ double red[STHR / 64];
     const JobDev& J = jobs[blockIdx.x];
     const int nt = J.ntile[KIND_X];
+    // each thread sums its strided partials in index order; the loads of
+    // eight of them are issued together
+    const double* __restrict__ part = J.part;
     double s = 0;
-    for (int i = threadIdx.x; i < nt; i += STHR) s += J.part[i];
+    int i = threadIdx.x;
+    for (; i + 7 * STHR < nt; i += 8 * STHR) {
+        double a[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[q] = part[i + q * STHR];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += a[q];
+    }
+    for (; i < nt; i += STHR) s += part[i];
     s = block_sum<STHR>(s, red);
     if (threadIdx.x == 0) {
         JobState* st = J.st;

@@ -16,7 +16,7 @@
 #include <stdint.h>
 #include <string.h>
 
-#include "../../include/dwhip.h"
+#include "dwhip.h"
 #include "dwhip_internal.h"
 
 #define OTHR 256

@@ -38,6 +38,12 @@ __host__ __device__ constexpr bool cmode_is_y(int m)
 #ifndef DWHIP_CMAXE
 #define DWHIP_CMAXE 20   // complex elements per thread per stage, Y/Z passes
 #endif
+#ifndef DWHIP_YMAXE
+#define DWHIP_YMAXE DWHIP_CMAXE  // Y passes
+#endif
+#ifndef DWHIP_ZMAXE
+#define DWHIP_ZMAXE DWHIP_CMAXE  // Z passes
+#endif
 #define DWHIP_MAX_RADIX 25
 
 extern "C" {

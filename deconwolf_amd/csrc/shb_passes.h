@@ -113,7 +113,8 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         const int h = J.h, nx = J.nx, ny = J.ny, hcs = J.hcs;
         const int row0 = lt * J.xrows;
         const int nrows = min(J.xrows, ny * J.nz - row0);
-        const int iter = (MODE == XM_UPDATE || MODE == XM_RATIO) ? J.st->iter : 0;
+        const DWHIP_GL JobState* __restrict__ st = gp(J.st);
+        const int iter = (MODE == XM_UPDATE || MODE == XM_RATIO) ? st->iter : 0;
         // the SHB ratio is 0 outside the image box and only y < N, z < P rows
         // of y are used: pass E reads and writes only those rows (the RL
         // ratio is 1e-6 outside the image, so RL writes every row)
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
             const int z0 = row0 / ny, y0 = row0 - z0 * ny;
             const int zl = (row0 + nrows - 1) / ny;
             if (z0 >= J.P || (z0 == zl && y0 >= J.N)) {
-                if (tid == 0) J.part[lt] = 0.0;
+                if (tid == 0) gp(J.part)[lt] = 0.0;
                 continue;
             }
         }
@@ -135,7 +136,8 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         g.nthr = XTHR;
         g.npmagic = fdiv_magic((uint32_t)nrows);
         const uint32_t hmagic = fdiv_magic((uint32_t)h);
-        const float2* __restrict__ C = J.C;
+        const DWHIP_GL float2* __restrict__ C = gp(J.C);
+        TwPtr __restrict__ rtw = gp(J.rtw);
 
         auto lds_src = [&](int pen, int i) { return lds[pen * h + i]; };
         auto lds_dst = [&](int pen, int i, float2 v) { lds[pen * h + i] = v; };
@@ -147,10 +149,10 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                     const int zz = row / ny;
                     if (zz >= J.P || row - zz * ny >= J.N) return make_float2(0.f, 0.f);
                 }
-                const float2* crow = C + (size_t)row * hcs;
-                return c2r_split(crow[i], crow[h - i], J.rtw[i]);
+                const DWHIP_GL float2* crow = C + (size_t)row * hcs;
+                return c2r_split(ldg(crow + i), ldg(crow + h - i), ldg(rtw + i));
             };
-            fft_tile<DWHIP_XMAXE, +1, false, false, true>(J.pxi, g, tid, lds, J.tw, split_src, lds_dst);
+            fft_tile<DWHIP_XMAXE, +1, false, DWHIP_RMAX_X, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
             __syncthreads();
         }
 
@@ -158,9 +160,12 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         // in natural order in LDS; loads of XCHUNK elements are issued together
         double acc = 0;
         const int nwork = nrows * h;
-        float* Uk = (iter & 1) ? J.U1 : J.U0;          // x_{k-2} in, x_k out (k > 0)
-        const float* Ukm1 = (iter & 1) ? J.U0 : J.U1;  // x_{k-1}
-        const float bg = J.st->bg;
+        DWHIP_GL float* Uk = gp((iter & 1) ? J.U1 : J.U0);          // x_{k-2} in, x_k out (k > 0)
+        const DWHIP_GL float* Ukm1 = gp((iter & 1) ? J.U0 : J.U1);  // x_{k-1}
+        DWHIP_GL float* U0 = gp(J.U0);
+        DWHIP_GL float* W = gp(J.W);
+        const DWHIP_GL float* im = gp(J.im);
+        const float bg = st->bg;
         for (int w0 = tid; w0 < nwork; w0 += XCHUNK * XTHR) {
             float2 a[XCHUNK], b[XCHUNK], c[XCHUNK], wv[XCHUNK];
             size_t vi[XCHUNK];
@@ -182,14 +187,14 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                 if (w0 + q * XTHR >= nwork) continue;
                 if (inv) a[q] = lds[w0 + q * XTHR];
                 if (MODE == XM_UPDATE) {
-                    b[q] = *reinterpret_cast<const float2*>(Ukm1 + vi[q]);
-                    if (!J.rl) c[q] = *reinterpret_cast<const float2*>(Uk + vi[q]);
-                    if (iter > 0 && J.has_w) wv[q] = *reinterpret_cast<const float2*>(J.W + vi[q]);
+                    b[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(Ukm1 + vi[q]));
+                    if (!J.rl) c[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(Uk + vi[q]));
+                    if (iter > 0 && J.has_w) wv[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(W + vi[q]));
                 } else if (MODE == XM_LOAD) {
-                    a[q] = *reinterpret_cast<const float2*>(J.U0 + vi[q]);
+                    a[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(U0 + vi[q]));
                 } else if (MODE == XM_RATIO) {
                     if (yy[q] < J.N && zz[q] < J.P) {
-                        const float* imrow = J.im + (size_t)J.M * (yy[q] + (size_t)J.N * zz[q]);
+                        const DWHIP_GL float* imrow = im + (size_t)J.M * (yy[q] + (size_t)J.N * zz[q]);
                         if (m0[q] < J.M) b[q].x = imrow[m0[q]];
                         if (m0[q] + 1 < J.M) b[q].y = imrow[m0[q] + 1];
                     }
@@ -201,14 +206,13 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                 if (w >= nwork) continue;
                 float2 out;
                 if (MODE == XM_PSF) {
-                    const JobState* st = J.st;
                     const int sy = (yy[q] + st->mid[1]) % ny, sz = (zz[q] + st->mid[2]) % J.nz;
                     float v[2] = {0.f, 0.f};
                     if (sy < J.pN && sz < J.pP) {
 #pragma unroll
                         for (int e = 0; e < 2; ++e) {
                             const int sx = (m0[q] + e + st->mid[0]) % nx;
-                            if (sx < J.pM) v[e] = J.psf[sx + (size_t)J.pM * (sy + (size_t)J.pN * sz)];
+                            if (sx < J.pM) v[e] = gp(J.psf)[sx + (size_t)J.pM * (sy + (size_t)J.pN * sz)];
                         }
                     }
                     out = make_float2(v[0], v[1]);
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                     out = a[q];
                 } else if (MODE == XM_STORE) {
                     out = a[q];
-                    *reinterpret_cast<float2*>(J.U0 + vi[q]) = out;
+                    stg(reinterpret_cast<DWHIP_GL float2*>(U0 + vi[q]), out);
                 } else if (MODE == XM_ONES) {
                     const bool inr = yy[q] < J.N && zz[q] < J.P;
                     out = make_float2(inr && m0[q] < J.M ? 1.f : 0.f, inr && m0[q] + 1 < J.M ? 1.f : 0.f);
@@ -226,7 +230,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                     const float sigma = 0.01f;
                     out.x = p1.x > sigma ? (float)(1.0 / p1.x) : 0.f;
                     out.y = p1.y > sigma ? (float)(1.0 / p1.y) : 0.f;
-                    *reinterpret_cast<float2*>(J.W + vi[q]) = out;
+                    stg(reinterpret_cast<DWHIP_GL float2*>(W + vi[q]), out);
                 } else if (MODE == XM_UPDATE && J.rl) {
                     // Richardson-Lucy (method_rl.c:85-97, 357-366): f = x_{k-1},
                     // x_k = x~ * (f * W), clamped to bg when bg > 0
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                             xk.x = xk.x < bg ? bg : xk.x;
                             xk.y = xk.y < bg ? bg : xk.y;
                         }
-                        *reinterpret_cast<float2*>(Uk + vi[q]) = xk;
+                        stg(reinterpret_cast<DWHIP_GL float2*>(Uk + vi[q]), xk);
                     }
                     out = xk;
                 } else if (MODE == XM_UPDATE) {
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                             xk.x = xk.x < bg ? bg : xk.x;
                             xk.y = xk.y < bg ? bg : xk.y;
                         }
-                        *reinterpret_cast<float2*>(Uk + vi[q]) = xk;
+                        stg(reinterpret_cast<DWHIP_GL float2*>(Uk + vi[q]), xk);
                     }
                     const double al = shb_alpha(iter, J.alphamax);
                     out = make_float2(shb_extrap(xk.x, xkm1.x, al, bg), shb_extrap(xk.y, xkm1.y, al, bg));
@@ -304,16 +308,16 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         }
         if (MODE == XM_RATIO) {
             const double s = block_sum<XTHR>(acc, red);
-            if (tid == 0) J.part[lt] = s;
+            if (tid == 0) gp(J.part)[lt] = s;
         }
         __syncthreads();
 
         if (fwd) {
-            fft_tile<DWHIP_XMAXE, -1, false, true, true>(J.pxf, g, tid, lds, J.tw, lds_src, lds_dst);
+            fft_tile<DWHIP_XMAXE, -1, false, DWHIP_RMAX_X, true, true>(J.pxf, g, tid, lds, gp(J.tw), lds_src, lds_dst);
             __syncthreads();
             const int hp = h / 2 + 1;
             const uint32_t hpmagic = fdiv_magic((uint32_t)hp);
-            float2* __restrict__ dst = J.C;
+            DWHIP_GL float2* __restrict__ dst = gp(J.C);
             for (int w = tid; w < nrows * hp; w += XTHR) {
                 const int b = (int)fdiv((uint32_t)w, hpmagic);
                 const int k = w - b * hp;
@@ -322,15 +326,15 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                     if (zz >= J.P || row - zz * ny >= J.N) continue;
                 }
                 const float2* L = lds + b * h;
-                float2* crow = dst + (size_t)(row0 + b) * hcs;
+                DWHIP_GL float2* crow = dst + (size_t)(row0 + b) * hcs;
                 if (k == 0) {
                     const float2 z0 = L[0];
-                    crow[0] = make_float2(z0.x + z0.y, 0.f);
-                    crow[h] = make_float2(z0.x - z0.y, 0.f);
+                    stg(crow, make_float2(z0.x + z0.y, 0.f));
+                    stg(crow + h, make_float2(z0.x - z0.y, 0.f));
                 } else {
                     const float2 a = L[k], c = L[h - k];
-                    crow[k] = r2c_merge(a, c, J.rtw[k]);
-                    if (2 * k != h) crow[h - k] = r2c_merge(c, a, J.rtw[h - k]);
+                    stg(crow + k, r2c_merge(a, c, ldg(rtw + k)));
+                    if (2 * k != h) stg(crow + h - k, r2c_merge(c, a, ldg(rtw + h - k)));
                 }
             }
         }
@@ -345,12 +349,12 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
 // fft_mul_conj_inplace (fft.c:490-512) with their operand order.
 template <bool CONJ>
 struct OtfOp {
-    const float2* __restrict__ K;
+    const DWHIP_GL float2* __restrict__ K;
     size_t stride;
     int nb;
     __device__ __forceinline__ float2 fetch(int pen, int i) const
     {
-        return pen < nb ? K[i * stride + pen] : make_float2(0.f, 0.f);
+        return pen < nb ? ldg(K + i * stride + pen) : make_float2(0.f, 0.f);
     }
     __device__ __forceinline__ float2 apply(float2 v, float2 kk) const
     {
@@ -380,7 +384,7 @@ constexpr int col_threads()
 // Y tiles (1024 threads, up to 159 KiB of LDS) need 4 waves per SIMD
 // when the kernel stays within 128 VGPRs.
 template <int MODE>
-__global__ __launch_bounds__(col_threads<MODE>()) __attribute__((amdgpu_waves_per_eu(cmode_is_y(MODE) ? 4 : 1)))
+__global__ __launch_bounds__(col_threads<MODE>()) __attribute__((amdgpu_waves_per_eu(4)))
 void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
 {
     extern __shared__ float2 lds[];
@@ -406,8 +410,8 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         const int nb = min(B, J.hc - kx0);   // columns past hc are row padding, never touched
         const size_t stride = kind == KIND_Y ? (size_t)hcs : (size_t)hcs * ny;
         const size_t off = kind == KIND_Y ? (size_t)outer * ny * hcs + kx0 : (size_t)outer * hcs + kx0;
-        const float2* __restrict__ src = J.C + off;
-        float2* __restrict__ dst = (MODE == CM_ZOTF ? J.K : J.C) + off;
+        const DWHIP_GL float2* __restrict__ src = gp(J.C) + off;
+        DWHIP_GL float2* __restrict__ dst = gp(MODE == CM_ZOTF ? J.K : J.C) + off;
         StageGeo g;
         g.n = kind == KIND_Y ? ny : J.nz;
         g.npen = B;
@@ -419,29 +423,29 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         const int nout = (MODE == CM_YINV_IMG) ? J.N : (MODE == CM_ZCONV) ? J.P : g.n;
 
         auto gsrc = [&](int pen, int i) {
-            return (pen < nb && i < nin) ? src[i * stride + pen] : make_float2(0.f, 0.f);
+            return (pen < nb && i < nin) ? ldg(src + i * stride + pen) : make_float2(0.f, 0.f);
         };
         auto gdst = [&](int pen, int i, float2 v) {
-            if (pen < nb && i < nout) dst[i * stride + pen] = v;
+            if (pen < nb && i < nout) stg(dst + i * stride + pen, v);
         };
         if (MODE == CM_YFWD || MODE == CM_YFWD_IMG) {
-            fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pyf, g, tid, lds, J.tw, gsrc, gdst);
+            fft_tile<DWHIP_YMAXE, -1, true, DWHIP_RMAX_Y, false, false>(J.pyf, g, tid, lds, gp(J.tw), gsrc, gdst);
         } else if (MODE == CM_YINV || MODE == CM_YINV_IMG) {
-            fft_tile<DWHIP_CMAXE, +1, true, false, false>(J.pyi, g, tid, lds, J.tw, gsrc, gdst);
+            fft_tile<DWHIP_YMAXE, +1, true, DWHIP_RMAX_Y, false, false>(J.pyi, g, tid, lds, gp(J.tw), gsrc, gdst);
         } else if (MODE == CM_ZFWD) {
-            fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pzf, g, tid, lds, J.tw, gsrc, gdst);
+            fft_tile<DWHIP_ZMAXE, -1, true, DWHIP_RMAX_Z, false, false>(J.pzf, g, tid, lds, gp(J.tw), gsrc, gdst);
         } else if (MODE == CM_ZINV) {
-            fft_tile<DWHIP_CMAXE, +1, true, false, false>(J.pzi, g, tid, lds, J.tw, gsrc, gdst);
+            fft_tile<DWHIP_ZMAXE, +1, true, DWHIP_RMAX_Z, false, false>(J.pzi, g, tid, lds, gp(J.tw), gsrc, gdst);
         } else if (MODE == CM_ZOTF) {
             // (float)nx*ny*nz, evaluated left to right in float (fft.c:856)
             const float scale = ((float)J.nx * (float)J.ny) * (float)J.nz;
             auto sdst = [&](int pen, int i, float2 v) {
-                if (pen < nb) dst[i * stride + pen] = make_float2(v.x / scale, v.y / scale);
+                if (pen < nb) stg(dst + i * stride + pen, make_float2(v.x / scale, v.y / scale));
             };
-            fft_tile<DWHIP_CMAXE, -1, true, false, false>(J.pzf, g, tid, lds, J.tw, gsrc, sdst);
+            fft_tile<DWHIP_ZMAXE, -1, true, DWHIP_RMAX_Z, false, false>(J.pzf, g, tid, lds, gp(J.tw), gsrc, sdst);
         } else {
-            OtfOp<MODE == CM_ZCONVJ> op{J.K + off, stride, nb};
-            fft_pair_tile<DWHIP_CMAXE, -1, +1, true>(J.pzf, J.pzi, g, tid, lds, J.tw, gsrc, op, gdst);
+            OtfOp<MODE == CM_ZCONVJ> op{gp(J.K) + off, stride, nb};
+            fft_pair_tile<DWHIP_ZMAXE, -1, +1, true, DWHIP_RMAX_Z>(J.pzf, J.pzi, g, tid, lds, gp(J.tw), gsrc, op, gdst);
         }
         __syncthreads();
     }
