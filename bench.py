@@ -28,6 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# iterations of the job the CPU baseline times (about 10 s on 16 host threads)
+CPU_SAMPLE_ITERS = 16
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "shb_c2_pmc_traffic.md")
 METRIC = "SHB voxel-iter/s batched-served vs plain per-image; p50/p99 job latency"
 
 
@@ -40,7 +44,7 @@ def run_cpu_baseline(npz, iters_full, threads, timeout_s):
     env = dict(os.environ)
     env["MKL_THREADING_LAYER"] = "GNU"
     env["OMP_NUM_THREADS"] = str(threads)
-    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--inputs", npz, "--iters", "2",
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--inputs", npz, "--iters", str(CPU_SAMPLE_ITERS),
            "--full-iters", str(iters_full), "--threads", str(threads)]
     t = time.time()
     try:
@@ -55,6 +59,19 @@ def run_cpu_baseline(npz, iters_full, threads, timeout_s):
     d = json.loads(line)
     d["wall_s"] = time.time() - t
     return d
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (scaled FETCH_SIZE + WRITE_SIZE), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            cols = [c.strip() for c in line.split("|")]
+            if len(cols) > 6 and cols[2] == kernel:
+                return float(cols[6])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def _pct(xs, p):
@@ -210,6 +227,7 @@ def main():
     def pct(p):
         return lat_sorted[min(len(lat_sorted) - 1, int(round(p / 100.0 * (len(lat_sorted) - 1))))]
 
+    traffic = profiled_traffic(dname) if a.config == "c2" else None
     served = None
     if a.stream > 0:
         served = run_stream(a.stream, a.rate)
@@ -245,7 +263,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/shb_c2_pmc_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
+                              if traffic else None,
             "bytes_per_launch": dby / dn,
             "avg_launch_us": 1e3 * dms / dn,
             "all_passes_GBps": (all_by / 1e9) / (all_ms / 1e3),
