@@ -103,8 +103,8 @@ struct Table {
     char* host = nullptr;  // pinned
     int cap = 0;
     int n = 0;
-    size_t lds[KIND_N] = {0, 0, 0};
-    int ntile[KIND_N] = {0, 0, 0};
+    size_t lds[KIND_N] = {};
+    int ntile[KIND_N] = {};
     std::vector<EngJob*> jobs;
     BatchDev* bd() const { return (BatchDev*)dev; }
     JobDev* jd() const { return (JobDev*)(dev + 64); }
@@ -174,7 +174,7 @@ int table_build(Table* t, const std::vector<EngJob*>& jobs, hipStream_t s)
 // One pass over a table, with the profiling hooks of the per-image path.
 int table_pass(const Table& t, bool xkind, int mode, hipStream_t s)
 {
-    const int kind = xkind ? KIND_X : (cmode_is_y(mode) ? KIND_Y : KIND_Z);
+    const int kind = xkind ? KIND_X : cmode_kind(mode);
     if (t.n == 0 || t.ntile[kind] == 0) return DWHIP_OK;
     const int slot = dwhip_prof_begin(s);
     hipError_t e = xkind ? dwhip_launch_xpass(mode, t.bd(), t.jd(), t.ntile[kind], t.lds[kind], s)

@@ -262,6 +262,9 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
     t->lds[KIND_X] = (size_t)t->xrows * h * sizeof(float2);
     t->lds[KIND_Y] = (size_t)t->ycols * ny * sizeof(float2);
     t->lds[KIND_Z] = (size_t)t->zcols * nz * sizeof(float2);
+    // the pruned y passes: all slabs until the job's image depth is known
+    t->ntile[KIND_YD] = t->ntile[KIND_YF] = t->ntile[KIND_Y];
+    t->lds[KIND_YD] = t->lds[KIND_YF] = t->lds[KIND_Y];
 }
 
 static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -342,6 +345,13 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.pM = (int)pM; d.pN = (int)pN; d.pP = (int)pP;
     d.refMNP = (double)ref[0] * (double)ref[1] * (double)ref[2];
     dwhip_tile_config(w, &j->tiles);
+    {
+        // pass D transforms slabs z < P; pass F too, except for RL, whose
+        // ratio is 1e-6 outside the image
+        const int nct = (d.hc + j->tiles.ycols - 1) / j->tiles.ycols;
+        j->tiles.ntile[KIND_YD] = nct * (int)P;
+        j->tiles.ntile[KIND_YF] = prm->method == DWHIP_METHOD_RL ? nct * (int)w[2] : nct * (int)P;
+    }
     for (int k = 0; k < KIND_N; ++k) d.ntile[k] = j->tiles.ntile[k];
     d.xrows = j->tiles.xrows;
     d.ycols = j->tiles.ycols;
@@ -543,10 +553,10 @@ int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s)
     RCHK(dwhip_pass_x(XM_UPDATE, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
     RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
     RCHK(dwhip_pass_c(CM_ZCONV, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
-    RCHK(dwhip_pass_c(CM_YINV_IMG, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+    RCHK(dwhip_pass_c(CM_YINV_IMG, j, j->bd, j->jd, t.ntile[KIND_YD], t.lds[KIND_YD], s));
     RCHK(dwhip_pass_x(XM_RATIO, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
     if (full) {
-        RCHK(dwhip_pass_c(CM_YFWD_IMG, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
+        RCHK(dwhip_pass_c(CM_YFWD_IMG, j, j->bd, j->jd, t.ntile[KIND_YF], t.lds[KIND_YF], s));
         RCHK(dwhip_pass_c(CM_ZCONVJ, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
         RCHK(dwhip_pass_c(CM_YINV, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
     }

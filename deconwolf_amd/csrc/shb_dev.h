@@ -10,7 +10,12 @@
 #include <stdint.h>
 #include "fft_engine.h"
 
-enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2, KIND_N = 3 };
+// Tile spaces: x-row tiles, y column tiles of all slabs, z column tiles,
+// and the y tiles of the pruned passes — KIND_YD: slabs z < P (pass D);
+// KIND_YF: pass F's slabs (z < P for SHB, all of them for RL).  A pruned
+// pass enumerates only the tiles it works on, so its grid carries no idle
+// workgroups and every XCD gets a share of the work.
+enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2, KIND_YD = 3, KIND_YF = 4, KIND_N = 5 };
 
 // metric / start values mirror dw_metric and dw_start_condition (src/dw.h:64-77)
 enum { DWHIP_DEV_METRIC_MSE = 0, DWHIP_DEV_METRIC_IDIV = 1 };

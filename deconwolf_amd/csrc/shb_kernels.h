@@ -19,6 +19,11 @@ __host__ __device__ constexpr bool cmode_is_y(int m)
 {
     return m == CM_YFWD || m == CM_YINV || m == CM_YINV_IMG || m == CM_YFWD_IMG;
 }
+// tile space of a column mode
+__host__ __device__ constexpr int cmode_kind(int m)
+{
+    return m == CM_YINV_IMG ? KIND_YD : m == CM_YFWD_IMG ? KIND_YF : cmode_is_y(m) ? KIND_Y : KIND_Z;
+}
 
 #ifndef DWHIP_XTHR
 #define DWHIP_XTHR 256   // threads per X-pass workgroup

@@ -389,31 +389,29 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
 {
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
-    constexpr int kind = cmode_is_y(MODE) ? KIND_Y : KIND_Z;
+    constexpr int kind = cmode_kind(MODE);
+    constexpr bool ykind = cmode_is_y(MODE);
     constexpr int NT = col_threads<MODE>();
     static_assert(MODE >= 0 && MODE < DWHIP_N_CMODES, "column mode");
     const int total = bd->ntile[kind];
     const int njobs = bd->njobs;
     for (int t0 = blockIdx.x; t0 < total; t0 += gridDim.x) {
         // XCD-aware order when the grid covers the tiles exactly once
-        const int t = (kind == KIND_Y && (int)gridDim.x == total) ? xcd_tile(t0, total) : t0;
+        const int t = (ykind && (int)gridDim.x == total) ? xcd_tile(t0, total) : t0;
         const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
         const int lt = t - J.tile0[kind];
         const int hcs = J.hcs, ny = J.ny;
-        const int B = kind == KIND_Y ? J.ycols : J.zcols;
+        const int B = ykind ? J.ycols : J.zcols;
         const int nct = (J.hc + B - 1) / B;
         const int outer = lt / nct;
-        // Y passes of the image chain touch only slabs z < P (the RL
-        // ratio is 1e-6 outside the image, so its adjoint is not pruned)
-        if ((MODE == CM_YINV_IMG || (MODE == CM_YFWD_IMG && !J.rl)) && outer >= J.P) continue;
         const int kx0 = (lt - outer * nct) * B;
         const int nb = min(B, J.hc - kx0);   // columns past hc are row padding, never touched
-        const size_t stride = kind == KIND_Y ? (size_t)hcs : (size_t)hcs * ny;
-        const size_t off = kind == KIND_Y ? (size_t)outer * ny * hcs + kx0 : (size_t)outer * hcs + kx0;
+        const size_t stride = ykind ? (size_t)hcs : (size_t)hcs * ny;
+        const size_t off = ykind ? (size_t)outer * ny * hcs + kx0 : (size_t)outer * hcs + kx0;
         const DWHIP_GL float2* __restrict__ src = gp(J.C) + off;
         DWHIP_GL float2* __restrict__ dst = gp(MODE == CM_ZOTF ? J.K : J.C) + off;
         StageGeo g;
-        g.n = kind == KIND_Y ? ny : J.nz;
+        g.n = ykind ? ny : J.nz;
         g.npen = B;
         g.pstride = 0;
         g.nthr = NT;
