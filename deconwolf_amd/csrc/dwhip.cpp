@@ -259,7 +259,9 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
     t->ntile[KIND_X] = (ny * nz + t->xrows - 1) / t->xrows;
     t->ntile[KIND_Y] = ((hc + t->ycols - 1) / t->ycols) * nz;
     t->ntile[KIND_Z] = ((hc + t->zcols - 1) / t->zcols) * ny;
-    t->lds[KIND_X] = (size_t)t->xrows * h * sizeof(float2);
+    // x tiles hold xrows rows of h values, or of hc values while the
+    // half-spectrum rows are staged
+    t->lds[KIND_X] = (size_t)t->xrows * (DWHIP_X_STAGE_LDS ? hc : h) * sizeof(float2);
     t->lds[KIND_Y] = (size_t)t->ycols * ny * sizeof(float2);
     t->lds[KIND_Z] = (size_t)t->zcols * nz * sizeof(float2);
     // the pruned y passes: all slabs until the job's image depth is known

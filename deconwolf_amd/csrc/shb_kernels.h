@@ -50,6 +50,9 @@ __host__ __device__ constexpr int cmode_kind(int m)
 #define DWHIP_ZMAXE DWHIP_CMAXE  // Z passes
 #endif
 #define DWHIP_MAX_RADIX 25
+#ifndef DWHIP_X_STAGE_LDS
+#define DWHIP_X_STAGE_LDS 1  // X passes stage their half-spectrum rows in LDS before the C2R
+#endif
 
 extern "C" {
 hipError_t dwhip_launch_xpass(int mode, const BatchDev* bd, const JobDev* jobs, int grid, size_t lds, hipStream_t s);

@@ -99,6 +99,11 @@ __device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
 
 #define XCHUNK 8  // elements per thread whose loads are issued together
 
+// Passes that stage their half-spectrum rows in LDS before the C2R (faster
+// for the ratio pass, whose tiles are otherwise bound by the latency of the
+// split's scattered loads; slower for the update pass).
+__host__ __device__ constexpr bool x_stage_rows(int mode) { return DWHIP_X_STAGE_LDS && mode == XM_RATIO; }
+
 template <int MODE>
 __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
 {
@@ -142,7 +147,46 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         auto lds_src = [&](int pen, int i) { return lds[pen * h + i]; };
         auto lds_dst = [&](int pen, int i, float2 v) { lds[pen * h + i] = v; };
 
-        if (inv) {
+        if (inv && x_stage_rows(MODE)) {
+            // The tile's half-spectrum rows (hc values each) go to LDS first,
+            // with coalesced loads of XCHUNK values per thread in flight; the
+            // split then reads both of its operands (k and h - k) from there.
+            // Rows the ratio pass skips are zeros.
+            const int hc = h + 1;
+            const uint32_t hcmagic = fdiv_magic((uint32_t)hc);
+            uint64_t live = ~0ull;  // bit b: row row0 + b is transformed
+            if (MODE == XM_RATIO) {
+                live = 0;
+                int z = row0 / ny, y = row0 - (row0 / ny) * ny;
+                for (int b = 0; b < nrows; ++b) {
+                    if (z < J.P && y < J.N) live |= 1ull << b;
+                    if (++y == ny) { y = 0; ++z; }
+                }
+            }
+            const int nraw = nrows * hc;
+            for (int w0 = tid; w0 < nraw; w0 += XCHUNK * XTHR) {
+                float2 v[XCHUNK];
+#pragma unroll
+                for (int q = 0; q < XCHUNK; ++q) {
+                    const int w = w0 + q * XTHR;
+                    v[q] = make_float2(0.f, 0.f);
+                    if (w < nraw) {
+                        const int b = (int)fdiv((uint32_t)w, hcmagic);
+                        if ((live >> b) & 1ull) v[q] = ldg(C + (size_t)(row0 + b) * hcs + (w - b * hc));
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < XCHUNK; ++q)
+                    if (w0 + q * XTHR < nraw) lds[w0 + q * XTHR] = v[q];
+            }
+            __syncthreads();
+            auto split_src = [&](int pen, int i) {
+                const float2* L = lds + pen * hc;
+                return c2r_split(L[i], L[h - i], ldg(rtw + i));
+            };
+            fft_tile<DWHIP_XMAXE, +1, false, DWHIP_RMAX_X, true, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
+            __syncthreads();
+        } else if (inv) {
             auto split_src = [&](int pen, int i) {
                 const int row = row0 + pen;
                 if (MODE == XM_RATIO) {
