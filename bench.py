@@ -28,6 +28,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# served stream: cap on the transform voxels (millions) iterating together
+SERVE_MVOX = 320
 # iterations of the job the CPU baseline times (about 10 s on 16 host threads)
 CPU_SAMPLE_ITERS = 16
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
@@ -86,6 +88,7 @@ def run_stream(n, rate):
     import torch
     import deconwolf_amd as dw
     from deconwolf_amd import workloads
+    from deconwolf_amd import _lib
     from deconwolf_amd.serve import Engine, serve
 
     t = time.time()
@@ -114,12 +117,19 @@ def run_stream(n, rate):
 
     free, total = torch.cuda.mem_get_info()
     budget = max(4.0, 0.8 * free / (1 << 30))
-    eng = Engine(hbm_budget_gb=budget, max_active=64)
-    reqs = [(i, j["im"], j["psf"], opts(j)) for i, j in enumerate(jobs)]
-    times = serve(eng, reqs, arrivals)
-    eng.close()
-    lat_s = [d - a for a, d in times.values()]
-    served_s = max(d for _, d in times.values()) - arrivals[0]
+
+    def run_engine(order, mvox):
+        eng = Engine(hbm_budget_gb=budget, max_active=64, order=order, active_mvox=mvox)
+        reqs = [(i, j["im"], j["psf"], opts(j)) for i, j in enumerate(jobs)]
+        times = serve(eng, reqs, arrivals)
+        eng.close()
+        lat = [d - a for a, d in times.values()]
+        return lat, max(d for _, d in times.values()) - arrivals[0]
+
+    # served: least work first, at most SERVE_MVOX transform voxels iterating
+    # together (a c2-sized job is 213 M); also FIFO with everything admitted
+    lat_s, served_s = run_engine(_lib.ORDER_SHORTEST, SERVE_MVOX)
+    lat_f, fifo_s = run_engine(_lib.ORDER_FIFO, 0)
     return {
         "workload": workloads.describe_stream(n, rate),
         "voxel_iters": work,
@@ -128,6 +138,9 @@ def run_stream(n, rate):
         "served_vs_plain": plain_s / served_s,
         "served_p50_ms": 1e3 * _pct(lat_s, 50), "served_p99_ms": 1e3 * _pct(lat_s, 99),
         "plain_p50_ms": 1e3 * _pct(lat_p, 50), "plain_p99_ms": 1e3 * _pct(lat_p, 99),
+        "served_policy": "least work first, at most %d M transform voxels iterating together" % SERVE_MVOX,
+        "served_fifo_all_value": work / fifo_s,
+        "served_fifo_all_p50_ms": 1e3 * _pct(lat_f, 50), "served_fifo_all_p99_ms": 1e3 * _pct(lat_f, 99),
         "engine_hbm_budget_gb": budget,
         "input_gen_s": t_gen,
     }
@@ -141,7 +154,7 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-timeout", type=int, default=240)
-    ap.add_argument("--stream", type=int, default=12, metavar="JOBS",
+    ap.add_argument("--stream", type=int, default=16, metavar="JOBS",
                     help="jobs of the mixed-size c4 request stream run after the c2 steps, served (batched "
                          "engine) and plain (one at a time); 0 skips it")
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate of the stream, jobs/s")

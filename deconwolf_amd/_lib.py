@@ -20,6 +20,8 @@ METRIC_MSE, METRIC_IDIV = 0, 1
 START_FLAT, START_LP, START_IDENTITY = 0, 1, 2
 # dwhip_to_u16_device modes
 U16_ROUND, U16_TRUNC, U16_RINT = 0, 1, 2
+# dwhip_engine_config.order
+ORDER_FIFO, ORDER_SHORTEST = 0, 1
 # dwhip_params.method
 DWHIP_METHOD_SHB, DWHIP_METHOD_RL = 0, 1
 # dw_method (src/dw.h:52-60), the values of DwOpts.method
@@ -61,7 +63,13 @@ class DwhipStats(ctypes.Structure):
 
 
 class EngineConfig(ctypes.Structure):
-    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+    _fields_ = [
+        ("hbm_budget", ctypes.c_size_t),
+        ("max_active", ctypes.c_int),
+        ("order", ctypes.c_int),
+        ("active_mvox", ctypes.c_int),
+        ("reserved", ctypes.c_int * 4),
+    ]
 
 
 class DwhipTile(ctypes.Structure):

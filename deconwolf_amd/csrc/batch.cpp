@@ -206,17 +206,49 @@ bool keep_going(const EngJob* j, int iter)
     return iter < iter_target(j);
 }
 
+double job_voxels(const EngJob* j)
+{
+    return (double)j->j.work[0] * (double)j->j.work[1] * (double)j->j.work[2];
+}
+
+double job_work(const EngJob* j)
+{
+    return job_voxels(j) * (double)iter_target(j);
+}
+
+// Admission: while the active set has room (max_active, the voxel cap and
+// the arena), the next job in the configured order that fits.  FIFO stops
+// at the first job that does not fit, so later jobs never overtake it;
+// shortest-first takes the least work among those that fit.
 int admit(dwhip_engine* E, std::vector<EngJob*>& fresh)
 {
+    const double cap = (double)E->cfg.active_mvox * 1e6;
+    double vox = 0;
+    for (EngJob* j : E->active) vox += job_voxels(j);
     while (!E->queue.empty() && (int)E->active.size() < E->cfg.max_active) {
-        EngJob* j = E->queue.front();
-        size_t off = 0;
-        if (!E->arena.alloc(j->j.bytes, &off)) break;
-        E->queue.pop_front();
-        j->off = off;
-        RCHK(dwhip_job_bind(&j->j, E->arena_mem + off, E->s));
-        E->active.push_back(j);
-        fresh.push_back(j);
+        std::vector<size_t> order(E->queue.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        if (E->cfg.order == DWHIP_ORDER_SHORTEST)
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t a, size_t b) { return job_work(E->queue[a]) < job_work(E->queue[b]); });
+        bool taken = false;
+        for (size_t i : order) {
+            EngJob* j = E->queue[i];
+            const bool fits_cap = cap <= 0 || E->active.empty() || vox + job_voxels(j) <= cap;
+            size_t off = 0;
+            if (fits_cap && E->arena.alloc(j->j.bytes, &off)) {
+                E->queue.erase(E->queue.begin() + i);
+                j->off = off;
+                RCHK(dwhip_job_bind(&j->j, E->arena_mem + off, E->s));
+                E->active.push_back(j);
+                fresh.push_back(j);
+                vox += job_voxels(j);
+                taken = true;
+                break;
+            }
+            if (E->cfg.order == DWHIP_ORDER_FIFO) break;
+        }
+        if (!taken) break;
     }
     return DWHIP_OK;
 }
@@ -262,6 +294,8 @@ void dwhip_engine_config_default(dwhip_engine_config* c)
 int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engine** out)
 {
     if (!cfg || !out || cfg->hbm_budget < ((size_t)1 << 20) || cfg->max_active < 1 || cfg->max_active > 4096)
+        return DWHIP_ERR_ARG;
+    if ((cfg->order != DWHIP_ORDER_FIFO && cfg->order != DWHIP_ORDER_SHORTEST) || cfg->active_mvox < 0)
         return DWHIP_ERR_ARG;
     int rc = dwhip_init();
     if (rc != DWHIP_OK) return rc;

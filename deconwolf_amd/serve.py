@@ -20,12 +20,18 @@ from .method_shb import DwOpts, _params
 
 
 class Engine:
-    def __init__(self, hbm_budget_gb=32.0, max_active=64, stream=None):
+    """order: _lib.ORDER_FIFO (arrival order) or ORDER_SHORTEST (least work
+    first); active_mvox: cap on the transform voxels (millions) of the jobs
+    iterating together, 0 for none."""
+
+    def __init__(self, hbm_budget_gb=32.0, max_active=64, stream=None, order=0, active_mvox=0):
         import torch
         cfg = EngineConfig()
         lib().dwhip_engine_config_default(ctypes.byref(cfg))
         cfg.hbm_budget = int(hbm_budget_gb * (1 << 30))
         cfg.max_active = int(max_active)
+        cfg.order = int(order)
+        cfg.active_mvox = int(active_mvox)
         h = ctypes.c_void_p()
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().dwhip_engine_create(ctypes.byref(cfg), s, ctypes.byref(h)), "dwhip_engine_create")

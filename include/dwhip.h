@@ -213,10 +213,17 @@ int dwhip_fft3_device(float *vol, float *spec, int64_t nx, int64_t ny, int64_t n
  */
 typedef struct dwhip_engine dwhip_engine;
 
+/* admission order of queued jobs */
+#define DWHIP_ORDER_FIFO 0      /* arrival order */
+#define DWHIP_ORDER_SHORTEST 1  /* least work (job voxels x iterations) first */
+
 typedef struct {
     size_t hbm_budget;  /* device bytes for job state (allocated at creation) */
     int max_active;     /* jobs iterating together */
-    int reserved[6];
+    int order;          /* DWHIP_ORDER_*; default FIFO */
+    int active_mvox;    /* cap on the transform voxels (millions) of the jobs iterating
+                           together, 0 = none; a job larger than the cap runs alone */
+    int reserved[4];
 } dwhip_engine_config;
 
 void dwhip_engine_config_default(dwhip_engine_config *cfg);

@@ -119,6 +119,54 @@ def test_rel_stopping_in_engine():
     eng.close()
 
 
+def test_shortest_first_and_voxel_cap():
+    """ORDER_SHORTEST admits the least work first; active_mvox caps the
+    transform voxels iterating together (a job above the cap runs alone);
+    results stay bit-identical to solo runs."""
+    from deconwolf_amd import _lib
+    specs = [(11, (16, 36, 40), (7, 9, 9), 9, {}), (12, (8, 20, 24), (5, 5, 5), 3, {}),
+             (13, (10, 24, 30), (5, 7, 7), 6, {})]
+    jobs = [make_job(sd, sh, ps, n, **o) for sd, sh, ps, n, o in specs]
+    work = []
+    for im, k, s in jobs:
+        w, _, _ = dw.work_size(tuple(im.shape), tuple(k.shape), 2)
+        work.append(w[0] * w[1] * w[2] * s.nIter)
+    # one job at a time: completion order = increasing work
+    eng = Engine(hbm_budget_gb=2.0, max_active=1, order=_lib.ORDER_SHORTEST)
+    for t, (im, k, s) in enumerate(jobs):
+        eng.submit(t, im, k, s)
+    order = []
+    while len(order) < len(jobs):
+        order += eng.step()
+        assert eng.status()[1] <= 1
+    assert order == sorted(range(len(jobs)), key=lambda t: work[t])
+    for t, (im, k, s) in enumerate(jobs):
+        ref, _ = solo(im, k, s)
+        out, _ = eng.result(t)
+        assert np.array_equal(out.cpu().numpy(), ref), t
+    eng.close()
+    # the voxel cap (millions of transform voxels): two 1.2-Mvox jobs never
+    # iterate together under a cap of 2, and do under a cap of 3
+    big = [make_job(21 + i, (40, 150, 150), (9, 9, 9), 3) for i in range(2)]
+    w, _, _ = dw.work_size((40, 150, 150), (9, 9, 9), 2)
+    assert 1_000_000 < w[0] * w[1] * w[2] < 1_500_000
+    for cap, most in ((2, 1), (3, 2)):
+        eng = Engine(hbm_budget_gb=2.0, max_active=8, active_mvox=cap)
+        for t, (im, k, s) in enumerate(big):
+            eng.submit(t, im, k, s)
+        n, seen = 0, 0
+        while n < len(big):
+            n += len(eng.step())
+            seen = max(seen, eng.status()[1])
+            assert eng.status()[1] <= most
+        assert cap == 2 or seen == 2
+        for t, (im, k, s) in enumerate(big):
+            ref, _ = solo(im, k, s)
+            out, _ = eng.result(t)
+            assert np.array_equal(out.cpu().numpy(), ref), t
+        eng.close()
+
+
 def test_oversized_job_refused():
     im, k, s = make_job(30, (32, 90, 100), (9, 9, 9), 2)   # about 9 MB of job state
     eng = Engine(hbm_budget_gb=0.002, max_active=2)
