@@ -107,6 +107,8 @@ SIGNATURES = [
     ("dwhip_output_stats_device", _int, [_vp, _i64, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_int64), _vp]),
     ("dwhip_to_u16_device", _int, [_vp, _i64, ctypes.c_float, _int, _vp, _vp]),
+    ("dwhip_bw_psf_device", _int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, _vp]),
     ("dwhip_fft3_device", _int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
     ("dwhip_engine_config_default", None, [ctypes.POINTER(EngineConfig)]),
     ("dwhip_engine_create", _int, [ctypes.POINTER(EngineConfig), _vp, ctypes.POINTER(_vp)]),

@@ -49,6 +49,7 @@ def units():
         u.append(("k_colpass.hip", ["-DCMODE=%d" % m], "k_colpass_%d.o" % m))
     u.append(("k_misc.hip", [], "k_misc.o"))
     u.append(("k_output.hip", [], "k_output.o"))
+    u.append(("k_bw.hip", [], "k_bw.o"))
     for src in ("dwhip.cpp", "profile.cpp", "batch.cpp", "tiling.cpp"):
         if os.path.exists(os.path.join(CSRC, src)):
             u.append((src, [], src.replace(".cpp", ".o")))

@@ -89,8 +89,9 @@ def describe_stream(n, rate):
 
 def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
     """n jobs of the c4 stream: list of dicts with im, psf (torch tensors on
-    device, prepared like dw_run), opts, arrival (s).  Blurring runs on the
-    GPU through torch.fft (input generation only)."""
+    device, prepared like dw_run), opts, arrival (s).  The PSFs come from
+    the device Born-Wolf generator; blurring runs on the GPU through
+    torch.fft (input generation only)."""
     import torch
     rng = np.random.default_rng(seed)
     xy = xy or C4["xy"]
@@ -105,7 +106,9 @@ def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
         lam = float(rng.choice(C4["lambdas"]))
         iters = int(rng.choice(C4["iters"]))
         if lam not in psfs:
-            psfs[lam] = psf_bw.bw_psf(pM, pN, pP, lam, NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=200.0)
+            # dw_bw's model on the GPU, normalised like bw_psf
+            raw = psf_bw.bw_psf_device(pN, pP, lam, NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=200.0).cpu().numpy()
+            psfs[lam] = fim_normalize_sum1(raw)
         psf = psfs[lam]
         nb = max(1, int(round(C4["blob_density"] * M * N * P)))
         V = synth.gaussian_blobs((P, N, M), nb, sigma=(1.0, 3.0), amp=(200.0, 4000.0), rng=rng)

@@ -183,6 +183,22 @@ int dwhip_to_u16_device(const float *v, int64_t n, float scaling, int mode, uint
                         void *stream);
 
 /*
+ * Born-Wolf widefield PSF on the device, the model of deconwolf's dw_bw
+ * (src/dw_bwpsf.c:538-775, src/bw_gsl.c:79-157): out receives N*N*P floats
+ * (x fastest), centred at ((N-1)/2, (N-1)/2, (P-1)/2), not normalised, for
+ * wavelength lambda_nm, numerical aperture NA, immersion index ni and pixel
+ * sizes dxy_nm (lateral) and dz_nm (axial).  The radial profile
+ * |int_0^1 J0(k0 NA r rho) exp(-i k0 NA^2 z rho^2 / (2 ni)) rho drho|^2 is
+ * sampled at 17 radii per pixel by Gauss-Legendre quadrature; each pixel is
+ * the mean over its square (8 x 8 Gauss-Legendre nodes) of the Lanczos-5
+ * interpolation of that profile, 0 beyond dw_bw's radmax; octant and z
+ * symmetry as dw_bw.  dw_bw integrates adaptively with GSL to 1e-6.
+ * Synchronises `stream`.
+ */
+int dwhip_bw_psf_device(float *out, int64_t N, int64_t P, double lambda_nm, double NA, double ni,
+                        double dxy_nm, double dz_nm, void *stream);
+
+/*
  * FFT check entry (device memory): unnormalised 3-D R2C of an nx*ny*nz
  * real volume (nx even) into (nx/2+1)*ny*nz complex, interleaved, the
  * layout of fftwf_plan_dft_r2c_3d(nz, ny, nx) that src/fft.c:236-290

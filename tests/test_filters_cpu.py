@@ -70,3 +70,23 @@ def test_bw_psf_symmetries():
     np.testing.assert_allclose(V, V[:, :, ::-1], rtol=1e-6)    # x
     np.testing.assert_allclose(V, np.transpose(V, (0, 2, 1)), rtol=1e-6)  # x <-> y
     assert np.unravel_index(np.argmax(V), V.shape) == (7, 15, 15)
+
+
+def test_lanczos5_interpolation():
+    """lanczos.c:102-160: samples are returned at integer positions (and
+    within 1e-9 of them), indices below 0 mirror, a smooth profile is
+    reproduced between samples."""
+    v = np.cos(0.05 * np.arange(60)) + 2.0
+    idx, w = psf_bw.lanczos5_weights(np.array([3.0, 7.0 + 5e-10, 12.0 - 5e-10]))
+    got = (v[idx] * w).sum(-1)
+    np.testing.assert_array_equal(got, [v[3], v[7], v[12]])
+    x = np.linspace(0.3, 40.7, 97)
+    idx, w = psf_bw.lanczos5_weights(x)
+    assert idx.min() >= 0 and np.all(idx[:, :5] == np.abs(np.floor(x)[:, None].astype(int) - np.arange(5)))
+    np.testing.assert_allclose((v[idx] * w).sum(-1), np.cos(0.05 * x) + 2.0, rtol=2e-3)
+
+
+def test_gauss_legendre_rule():
+    x, w = psf_bw.gauss_legendre01(12)
+    for k in range(24):   # exact for polynomials up to degree 23
+        assert abs((w * x ** k).sum() - 1.0 / (k + 1)) < 1e-13
