@@ -82,9 +82,11 @@ def deconvolve_tiles(im, psf, s: DwOpts, max_size, overlap, engine=None):
         opts.append(copy.copy(s))
     outs = []
     if engine is None:
-        from .method_shb import deconvolve_shb_hip_device
+        from ._lib import METHOD_RL
+        from .method_shb import deconvolve_rl_hip_device, deconvolve_shb_hip_device
+        run = deconvolve_rl_hip_device if int(s.method) == METHOD_RL else deconvolve_shb_hip_device
         for x, p, o in zip(ins, psfs, opts):
-            outs.append(deconvolve_shb_hip_device(x, p, o))
+            outs.append(run(x, p, o))
     else:
         base = 1 << 40
         for i, (x, p, o) in enumerate(zip(ins, psfs, opts)):

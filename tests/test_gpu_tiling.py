@@ -49,12 +49,13 @@ def _image(seed=1, shape=(10, 70, 84)):
     return im, k
 
 
-@pytest.mark.parametrize("use_engine", [False, True])
-def test_tiled_shb_matches_oracle(use_engine):
-    print("tiled shb, engine =", use_engine, flush=True)
+@pytest.mark.parametrize("use_engine,method", [(False, dw.METHOD_SHB), (True, dw.METHOD_SHB),
+                                               (False, dw.METHOD_RL), (True, dw.METHOD_RL)])
+def test_tiled_shb_matches_oracle(use_engine, method):
+    print("tiled, engine =", use_engine, "method", method, flush=True)
     im, k = _image()
     P, N, M = im.shape
-    s = dw.DwOpts(nIter=8, iter_type=dw.ITER_FIXED)
+    s = dw.DwOpts(nIter=8, iter_type=dw.ITER_FIXED, method=method)
     psf = tiling.tile_psf(k, im.shape, s)     # dw_run's preparation for the whole image
     eng = Engine(hbm_budget_gb=1.0, max_active=16) if use_engine else None
     out, tiles, opts = tiling.deconvolve_tiles(torch.from_numpy(im).to(DEV), psf, s, 40, 8, engine=eng)
@@ -67,7 +68,8 @@ def test_tiled_shb_matches_oracle(use_engine):
         x = to.get_tile(rt, im) + np.float32(s.offset)
         tp = tiling.tile_psf(psf, x.shape, s)
         w, _, _ = dw.work_size(x.shape, tp.shape)
-        o, _, _, _ = po.shb(x, tp, niter=8, work=w)
+        run = po.rl if method == dw.METHOD_RL else po.shb
+        o, _, _, _ = run(x, tp, niter=8, work=w)
         refs.append(to.finish_tile(o, s.offset))
     want = to.put_tiles(ref_tiles, refs, im.shape)
     rel = np.linalg.norm(out.astype(np.float64) - want) / np.linalg.norm(want)
