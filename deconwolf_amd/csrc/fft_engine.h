@@ -569,9 +569,15 @@ struct NoOp {
 // keep them within 128 VGPRs).
 #define DWHIP_FOR_RADICES(X) \
     X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(15) X(16) X(20) X(25)
+#ifndef DWHIP_RMAX_X
 #define DWHIP_RMAX_X 25
+#endif
+#ifndef DWHIP_RMAX_Y
 #define DWHIP_RMAX_Y 16
+#endif
+#ifndef DWHIP_RMAX_Z
 #define DWHIP_RMAX_Z 7
+#endif
 __host__ __device__ constexpr bool radix_ok(int rmax, int R) { return R <= rmax; }
 #define DWHIP_MAX_FFT_STAGES 4
 
