@@ -193,3 +193,25 @@ def test_shb_lp_start_matches_oracle():
     lp = filters.gsmooth(im, 8.0)
     o32, _, _, _ = po.shb(im, k, niter=6, start=po.START_LP, start_image=lp, work=s.stats["work"])
     assert rel_l2(out, o32) <= 1e-5
+
+
+@pytest.mark.parametrize("shape,psfshape", [((1, 40, 50), (1, 9, 9)),     # 2-D image: z transforms of length 1
+                                            ((6, 20, 24), (5, 31, 31)),   # PSF wider than the image
+                                            ((9, 1, 64), (5, 1, 7)),      # one row per plane
+                                            ((3, 17, 1), (3, 5, 1))])     # one column: x transform of length 2
+def test_shb_degenerate_shapes(shape, psfshape):
+    print("degenerate", shape, psfshape, flush=True)
+    rng = np.random.default_rng(31)
+    P, N, M = shape
+    im = (rng.random(shape) * 200 + 20).astype(np.float32)
+    k = rng.random(psfshape).astype(np.float32) + 0.05
+    k[tuple(d // 2 for d in psfshape)] += 2
+    k = (k / k.sum(dtype=np.float64)).astype(np.float32)
+    for method in (dw.METHOD_SHB, dw.METHOD_RL):
+        s = _opts(6, method=method)
+        run = dw.deconvolve_rl_hip if method == dw.METHOD_RL else dw.deconvolve_shb_hip
+        out = run(im, k, s)
+        ref = po.rl if method == dw.METHOD_RL else po.shb
+        o32, e32, _, _ = ref(im, k, niter=6, work=s.stats["work"])
+        assert rel_l2(out, o32) <= 1e-5, (method, rel_l2(out, o32))
+        np.testing.assert_allclose(s.err_trace, e32, rtol=1e-4)
