@@ -146,6 +146,40 @@ def run_stream(n, rate):
     }
 
 
+def run_tiled():
+    """c3: the 2048x2048x60 image in 16 overlapping tiles (tiling.c), the
+    tiles batched through the engine against the same tiles run one after
+    the other through the plain path; image voxel-iterations per second of
+    the whole tiled deconvolution (extraction, all tiles, blend)."""
+    import torch
+    from deconwolf_amd import tiling, workloads
+    from deconwolf_amd.serve import Engine
+
+    t = time.time()
+    im, psf, s = workloads.make_c3()
+    torch.cuda.synchronize()
+    t_gen = time.time() - t
+    c = workloads.C3
+    P, N, M = im.shape
+    work = M * N * P * s.nIter
+    res = {"workload": workloads.describe_c3(), "input_gen_s": t_gen}
+    for name, eng in (("plain_tiles", None), ("engine_tiles", "engine")):
+        e = Engine(hbm_budget_gb=64.0, max_active=64) if eng else None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out, tiles, _ = tiling.deconvolve_tiles(im, psf, s, c["tile"], c["overlap"], engine=e)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        if e is not None:
+            e.close()
+        res[name + "_value"] = work / el
+        res[name + "_s"] = el
+        res["tiles"] = len(tiles)
+        del out
+    res["engine_vs_plain"] = res["engine_tiles_value"] / res["plain_tiles_value"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +192,7 @@ def main():
                     help="jobs of the mixed-size c4 request stream run after the c2 steps, served (batched "
                          "engine) and plain (one at a time); 0 skips it")
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate of the stream, jobs/s")
+    ap.add_argument("--no-tiled", action="store_true", help="skip the c3 tiled run")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy a GIB-sized float32 buffer on the GPU 3 times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -244,6 +279,7 @@ def main():
     served = None
     if a.stream > 0:
         served = run_stream(a.stream, a.rate)
+    tiled = None if a.no_tiled else run_tiled()
 
     res = {
         "metric": METRIC,
@@ -285,6 +321,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "served": served,
+        "tiled": tiled,
         "kernels": kernels,
         "input_gen_s": t_gen,
     }

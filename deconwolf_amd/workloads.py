@@ -130,3 +130,48 @@ def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
         jobs.append(dict(im=im, psf=torch.as_tensor(ps, device=device).contiguous(), opts=s, arrival=t,
                          shape=(M, N, P), iters=iters, lam=lam))
     return jobs
+
+
+# ---------------------------------------------------------------------------
+# c3: one large image in overlapping tiles
+# ---------------------------------------------------------------------------
+C3 = dict(shape=(60, 2048, 2048), n=8000, sigma=(1.0, 3.0), amp=(200.0, 4000.0), bg=120.0, psf=(181, 181, 79),
+          lambda_nm=520.0, iters=50, tile=512, overlap=48, seed=3)
+
+
+def describe_c3():
+    c = C3
+    P, N, M = c["shape"]
+    return ("c3: %dx%dx%d float32, %d Gaussian blobs, background %g, Born-Wolf PSF 181x181x79 (%g nm), Poisson "
+            "noise, %d SHB iterations, tiles of %d px with %d px overlap (tiling.c)"
+            % (M, N, P, c["n"], c["bg"], c["lambda_nm"], c["iters"], c["tile"], c["overlap"]))
+
+
+def make_c3(device="cuda:0"):
+    """(im GPU tensor without offset, psf numpy normalised and autocropped for
+    the whole image, DwOpts).  PSF from the device generator, blur on the
+    GPU with torch.fft (input generation only)."""
+    import torch
+    c = C3
+    rng = np.random.default_rng(c["seed"])
+    pM, pN, pP = c["psf"]
+    raw = psf_bw.bw_psf_device(pN, pP, c["lambda_nm"], NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=200.0).cpu().numpy()
+    psf = fim_normalize_sum1(raw)
+    V = synth.gaussian_blobs(c["shape"], c["n"], sigma=c["sigma"], amp=c["amp"], rng=rng)
+    P, N, M = c["shape"]
+    full = (P + pP - 1, N + pN - 1, M + pM - 1)
+    a = torch.as_tensor(V.astype(np.float32), device=device)
+    del V
+    b = torch.as_tensor(psf, device=device)
+    B = torch.fft.irfftn(torch.fft.rfftn(a, full) * torch.fft.rfftn(b, full), full)
+    del a
+    cz, cy, cx = (pP - 1) // 2, (pN - 1) // 2, (pM - 1) // 2
+    B = B[cz:cz + P, cy:cy + N, cx:cx + M].clamp_min(0) + c["bg"]
+    g = torch.Generator(device=device)
+    g.manual_seed(c["seed"])
+    im = torch.poisson(B, generator=g).float().contiguous()
+    del B
+    s = DwOpts(nIter=c["iters"], iter_type=2)
+    ps = psf_autocrop(psf, (P, N, M), s)
+    ps = fim_normalize_sum1(ps)
+    return im, ps, s
