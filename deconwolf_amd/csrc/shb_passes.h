@@ -99,6 +99,11 @@ __device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
 
 #define XCHUNK 8  // elements per thread whose loads are issued together
 
+template <bool B>
+struct BoolC {
+    static constexpr bool value = B;
+};
+
 // Passes that stage their half-spectrum rows in LDS before the C2R (faster
 // for the ratio pass, whose tiles are otherwise bound by the latency of the
 // split's scattered loads; slower for the update pass).
@@ -210,7 +215,13 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
         DWHIP_GL float* W = gp(J.W);
         const DWHIP_GL float* im = gp(J.im);
         const float bg = st->bg;
-        for (int w0 = tid; w0 < nwork; w0 += XCHUNK * XTHR) {
+        // alpha of iterations k-1 and k (uniform)
+        const double am1 = MODE == XM_UPDATE ? shb_alpha(iter - 1, J.alphamax) : 0.0;
+        const double al = MODE == XM_UPDATE ? shb_alpha(iter, J.alphamax) : 0.0;
+        // chunks of XCHUNK elements per thread; full chunks carry no bounds
+        // tests, so their loads and arithmetic schedule as straight-line code
+        auto chunk = [&](int w0, auto full) {
+            using FULL = decltype(full);
             float2 a[XCHUNK], b[XCHUNK], c[XCHUNK], wv[XCHUNK];
             size_t vi[XCHUNK];
             int yy[XCHUNK], zz[XCHUNK], m0[XCHUNK];
@@ -228,7 +239,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
             }
 #pragma unroll
             for (int q = 0; q < XCHUNK; ++q) {
-                if (w0 + q * XTHR >= nwork) continue;
+                if (!FULL::value && w0 + q * XTHR >= nwork) continue;
                 if (inv) a[q] = lds[w0 + q * XTHR];
                 if (MODE == XM_UPDATE) {
                     b[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(Ukm1 + vi[q]));
@@ -247,7 +258,7 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
 #pragma unroll
             for (int q = 0; q < XCHUNK; ++q) {
                 const int w = w0 + q * XTHR;
-                if (w >= nwork) continue;
+                if (!FULL::value && w >= nwork) continue;
                 float2 out;
                 if (MODE == XM_PSF) {
                     const int sy = (yy[q] + st->mid[1]) % ny, sz = (zz[q] + st->mid[2]) % J.nz;
@@ -302,7 +313,6 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                         // finish iteration k-1 (method_shb.c:55-70, 357-376)
                         const float2 xt = a[q];
                         const float2 xkm2 = c[q];
-                        const double am1 = shb_alpha(iter - 1, J.alphamax);
                         const float p0 = shb_extrap(xkm1.x, xkm2.x, am1, bg);
                         const float p1 = shb_extrap(xkm1.y, xkm2.y, am1, bg);
                         if (J.has_w)
@@ -315,7 +325,6 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                         }
                         stg(reinterpret_cast<DWHIP_GL float2*>(Uk + vi[q]), xk);
                     }
-                    const double al = shb_alpha(iter, J.alphamax);
                     out = make_float2(shb_extrap(xk.x, xkm1.x, al, bg), shb_extrap(xk.y, xkm1.y, al, bg));
                 } else {  // XM_RATIO, method_shb.c:27-48 and getError dw.c:1436-1524
                     const float2 yv = a[q];
@@ -349,7 +358,10 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                 }
                 lds[w] = out;
             }
-        }
+                };
+        int w0 = tid;
+        for (; w0 + (XCHUNK - 1) * XTHR < nwork; w0 += XCHUNK * XTHR) chunk(w0, BoolC<true>());
+        for (; w0 < nwork; w0 += XCHUNK * XTHR) chunk(w0, BoolC<false>());
         if (MODE == XM_RATIO) {
             const double s = block_sum<XTHR>(acc, red);
             if (tid == 0) gp(J.part)[lt] = s;
