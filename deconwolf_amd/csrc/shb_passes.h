@@ -97,7 +97,9 @@ __device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
     return make_float2(s.x + e.y, s.y - e.x);        // s - i*e
 }
 
+#ifndef XCHUNK
 #define XCHUNK 8  // elements per thread whose loads are issued together
+#endif
 
 template <bool B>
 struct BoolC {
