@@ -342,16 +342,23 @@ __global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd,
                                 float y = e ? yv.y : yv.x;
                                 const float gv = e ? b[q].y : b[q].x;
                                 const double yd = y, gd = gv;
+                                // g / y of two floats: the correctly rounded float
+                                // quotient equals the double quotient rounded to
+                                // float (53 >= 2 * 24 + 2), so get_fIdiv's
+                                // logf(gval / yval) over doubles needs no double
+                                // division, and the ratio reuses it
+                                const float q = gv / y;
                                 if (J.metric == DWHIP_DEV_METRIC_IDIV) {
-                                    if (yd > 0 && gd > 0) acc += gd * (double)logf((float)(gd / yd)) - (gd - yd);
+                                    if (yd > 0 && gd > 0) acc += gd * (double)logf(q) - (gd - yd);
                                 } else {
                                     acc += (yd - gd) * (yd - gd);
                                 }
                                 if (J.rl) {
-                                    r[e] = y > 0 ? gv / y : bg;  // method_rl.c:52-60
+                                    r[e] = y > 0 ? q : bg;  // method_rl.c:52-60
+                                } else if (fabs(yd) < 1e-6) {
+                                    r[e] = gv / (float)copysign(1e-6, yd);  // method_shb.c:41-42
                                 } else {
-                                    if (fabs(yd) < 1e-6) y = (float)copysign(1e-6, yd);
-                                    r[e] = gv / y;
+                                    r[e] = q;
                                 }
                             }
                         }
