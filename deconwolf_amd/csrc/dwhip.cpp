@@ -250,7 +250,7 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
 {
     const int h = (int)(w[0] / 2), ny = (int)w[1], nz = (int)w[2];
     const int hc = h + 1;
-    t->xrows = std::max(1, std::min(64, DWHIP_XMAXE * DWHIP_XTHR / h));
+    t->xrows = std::max(1, std::min(DWHIP_XROWS_MAX, DWHIP_XMAXE * DWHIP_XTHR / h));
     // a column tile holds B pencils of n complex values in LDS
     const int ylim = std::min(DWHIP_YMAXE * DWHIP_CTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
     const int zlim = std::min(DWHIP_ZMAXE * DWHIP_ZTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));

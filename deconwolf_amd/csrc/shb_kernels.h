@@ -49,6 +49,9 @@ __host__ __device__ constexpr int cmode_kind(int m)
 #ifndef DWHIP_ZMAXE
 #define DWHIP_ZMAXE DWHIP_CMAXE  // Z passes
 #endif
+#ifndef DWHIP_XROWS_MAX
+#define DWHIP_XROWS_MAX 64  // rows per X tile at most
+#endif
 #define DWHIP_MAX_RADIX 25
 #ifndef DWHIP_X_STAGE_LDS
 #define DWHIP_X_STAGE_LDS 1  // X passes stage their half-spectrum rows in LDS before the C2R

@@ -111,8 +111,14 @@ struct BoolC {
 // split's scattered loads; slower for the update pass).
 __host__ __device__ constexpr bool x_stage_rows(int mode) { return DWHIP_X_STAGE_LDS && mode == XM_RATIO; }
 
+#ifdef DWHIP_X_WAVES
+#define DWHIP_X_OCC __attribute__((amdgpu_waves_per_eu(DWHIP_X_WAVES)))
+#else
+#define DWHIP_X_OCC
+#endif
+
 template <int MODE>
-__global__ __launch_bounds__(XTHR) void k_xpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
+__global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
 {
     extern __shared__ float2 lds[];
     __shared__ double red[XTHR / 64];
