@@ -311,28 +311,104 @@ __device__ __forceinline__ void stg(DWHIP_GL float2* p, float2 v)
 // ---------------------------------------------------------------------------
 struct StageGeo {
     int n;        // pencil length
-    int npen;     // pencils in the tile
+    int npen;     // pencils in the tile (a power of two for column tiles)
     int pstride;  // LDS pencil stride (rows layout)
     int nthr;
-    uint32_t npmagic;  // fdiv magic of npen
+    int npshift;  // log2(npen) for column tiles
 };
-
-template <bool COLS>
-__device__ __forceinline__ int lidx(const StageGeo& g, int pen, int i)
-{
-    return COLS ? i * g.npen + pen : pen * g.pstride + i;
-}
 
 // butterfly (pen, j) of work item w for a stage with nb butterflies per pencil
 template <bool COLS>
 __device__ __forceinline__ void bfly_of(const StageGeo& g, int w, int nb, uint32_t nbmagic, int& pen, int& j)
 {
     if (COLS) {
-        j = (int)fdiv((uint32_t)w, g.npmagic);
-        pen = w - j * g.npen;
+        j = w >> g.npshift;
+        pen = w & (g.npen - 1);
     } else {
         pen = (int)fdiv((uint32_t)w, nbmagic);
         j = w - pen * nb;
+    }
+}
+
+// LDS indices, strength-reduced: element r of a butterfly sits at
+// base + r * step.  All LDS indices are below 2^24 (at most 159 KiB of
+// float2), so products use the full-rate 24-bit multiply.
+//   in:   elements j + r*nb of butterfly (pen, j), work item w
+//   outR: elements j*R + r (first stage, junction)
+//   outM: elements (j/ns)*ns*R + j%ns + r*ns (middle stages), jq = j/ns
+template <bool COLS>
+__device__ __forceinline__ int lds_in(const StageGeo& g, int w, int pen, int j)
+{
+    return COLS ? w : (int)__umul24((uint32_t)pen, (uint32_t)g.pstride) + j;
+}
+template <bool COLS>
+__device__ __forceinline__ int lds_in_step(const StageGeo& g, int nb)
+{
+    return COLS ? nb * g.npen : nb;
+}
+template <int R, bool COLS>
+__device__ __forceinline__ int lds_outR(const StageGeo& g, int w, int pen, int j)
+{
+    return COLS ? w + (int)__umul24((uint32_t)j, (uint32_t)((R - 1) * g.npen))
+                : (int)__umul24((uint32_t)pen, (uint32_t)g.pstride) + j * R;
+}
+template <bool COLS>
+__device__ __forceinline__ int lds_outR_step(const StageGeo& g)
+{
+    return COLS ? g.npen : 1;
+}
+template <int R, bool COLS>
+__device__ __forceinline__ int lds_outM(const StageGeo& g, int w, int pen, int j, int jq, int ns)
+{
+    return COLS ? w + (int)__umul24((uint32_t)jq, (uint32_t)((R - 1) * ns * g.npen))
+                : (int)__umul24((uint32_t)pen, (uint32_t)g.pstride) + j + (int)__umul24((uint32_t)jq, (uint32_t)((R - 1) * ns));
+}
+template <bool COLS>
+__device__ __forceinline__ int lds_outM_step(const StageGeo& g, int ns)
+{
+    return COLS ? ns * g.npen : ns;
+}
+
+// Global operands.  A functor with `static constexpr bool kStrided = true`
+// moves the R elements j + r*nb of a butterfly in one call (load<R>,
+// store<R>, fetch<R>), so that it can form one address per butterfly and
+// step from there; any other functor is called per element.
+template <class F, class = void>
+struct io_strided {
+    static constexpr bool value = false;
+};
+template <class F>
+struct io_strided<F, decltype((void)F::kStrided, void())> {
+    static constexpr bool value = F::kStrided;
+};
+template <int R, class Src>
+__device__ __forceinline__ void io_load(const Src& src, int pen, int j, int nb, float2* v)
+{
+    if constexpr (io_strided<Src>::value) {
+        src.template load<R>(pen, j, nb, v);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = src(pen, j + r * nb);
+    }
+}
+template <int R, class Dst>
+__device__ __forceinline__ void io_store(const Dst& dst, int pen, int j, int nb, const float2* v)
+{
+    if constexpr (io_strided<Dst>::value) {
+        dst.template store<R>(pen, j, nb, v);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst(pen, j + r * nb, v[r]);
+    }
+}
+template <int R, class Op>
+__device__ __forceinline__ void io_fetch(const Op& op, int pen, int j, int nb, float2* v)
+{
+    if constexpr (io_strided<Op>::value) {
+        op.template fetch_r<R>(pen, j, nb, v);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = op.fetch(pen, j + r * nb);
     }
 }
 
@@ -353,9 +429,9 @@ __device__ __forceinline__ void twiddle(float2* v, TwPtr __restrict__ tw, int k)
 //         v[r] is element j + r*nb of pencil pen, in natural order
 // ---------------------------------------------------------------------------
 
-// first stage (ns = 1): global/generated input -> DFT -> LDS (or Dst)
-template <int R, int MB, int DIR, bool COLS, bool SYNC, class Src, class Dst>
-__device__ __forceinline__ void stage_first(const StageGeo& g, int tid, Src src, Dst dst)
+// first stage (ns = 1): global/generated input -> DFT -> LDS
+template <int R, int MB, int DIR, bool COLS, bool SYNC, class Src>
+__device__ __forceinline__ void stage_first(const StageGeo& g, int tid, Src src, float2* lds)
 {
     const int nb = g.n / R;
     const uint32_t nbm = fdiv_magic((uint32_t)nb);
@@ -367,11 +443,11 @@ __device__ __forceinline__ void stage_first(const StageGeo& g, int tid, Src src,
         if (w < total) {
             int pen, j;
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = src(pen, j + r * nb);
+            io_load<R>(src, pen, j, nb, v[q]);
         }
     }
     if (SYNC) __syncthreads();
+    const int ostep = lds_outR_step<COLS>(g);
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
         const int w = tid + q * g.nthr;
@@ -379,8 +455,9 @@ __device__ __forceinline__ void stage_first(const StageGeo& g, int tid, Src src,
             int pen, j;
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
             DftC<R, DIR>::run(v[q]);
+            float2* o = lds + lds_outR<R, COLS>(g, w, pen, j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst(pen, j * R + r, v[q][r]);
+            for (int r = 0; r < R; ++r) o[r * ostep] = v[q][r];
         }
     }
 }
@@ -393,6 +470,7 @@ __device__ __forceinline__ void stage_mid(const StageGeo& g, int tid, float2* ld
     const int nb = g.n / R;
     const uint32_t nbm = fdiv_magic((uint32_t)nb);
     const int total = nb * g.npen;
+    const int istep = lds_in_step<COLS>(g, nb);
     float2 v[MB][R];
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
@@ -400,11 +478,13 @@ __device__ __forceinline__ void stage_mid(const StageGeo& g, int tid, float2* ld
         if (w < total) {
             int pen, j;
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
+            const float2* in = lds + lds_in<COLS>(g, w, pen, j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = lds[lidx<COLS>(g, pen, j + r * nb)];
+            for (int r = 0; r < R; ++r) v[q][r] = in[r * istep];
         }
     }
     __syncthreads();
+    const int ostep = lds_outM_step<COLS>(g, ns);
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
         const int w = tid + q * g.nthr;
@@ -415,9 +495,9 @@ __device__ __forceinline__ void stage_mid(const StageGeo& g, int tid, float2* ld
             const int k = j - jq * ns;
             twiddle<R, DIR>(v[q], tw, k);
             DftC<R, DIR>::run(v[q]);
-            const int base = jq * ns * R + k;
+            float2* o = lds + lds_outM<R, COLS>(g, w, pen, j, jq, ns);
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lidx<COLS>(g, pen, base + r * ns)] = v[q][r];
+            for (int r = 0; r < R; ++r) o[r * ostep] = v[q][r];
         }
     }
     __syncthreads();
@@ -432,6 +512,7 @@ __device__ __forceinline__ void stage_last(const StageGeo& g, int tid, const flo
     const int nb = g.n / R;
     const uint32_t nbm = fdiv_magic((uint32_t)nb);
     const int total = nb * g.npen;
+    const int istep = lds_in_step<COLS>(g, nb);
     float2 v[MB][R];
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
@@ -439,8 +520,9 @@ __device__ __forceinline__ void stage_last(const StageGeo& g, int tid, const flo
         if (w < total) {
             int pen, j;
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
+            const float2* in = lds + lds_in<COLS>(g, w, pen, j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = lds[lidx<COLS>(g, pen, j + r * nb)];
+            for (int r = 0; r < R; ++r) v[q][r] = in[r * istep];
         }
     }
     if (SYNC) __syncthreads();
@@ -452,8 +534,7 @@ __device__ __forceinline__ void stage_last(const StageGeo& g, int tid, const flo
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
             twiddle<R, DIR>(v[q], tw, j);
             DftC<R, DIR>::run(v[q]);
-#pragma unroll
-            for (int r = 0; r < R; ++r) dst(pen, j + r * nb, v[q][r]);
+            io_store<R>(dst, pen, j, nb, v[q]);
         }
     }
 }
@@ -469,6 +550,7 @@ __device__ __forceinline__ void stage_junction(const StageGeo& g, int tid, float
     const int nb = g.n / R;
     const uint32_t nbm = fdiv_magic((uint32_t)nb);
     const int total = nb * g.npen;
+    const int istep = lds_in_step<COLS>(g, nb);
     float2 v[MB][R];
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
@@ -476,11 +558,13 @@ __device__ __forceinline__ void stage_junction(const StageGeo& g, int tid, float
         if (w < total) {
             int pen, j;
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
+            const float2* in = lds + lds_in<COLS>(g, w, pen, j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = lds[lidx<COLS>(g, pen, j + r * nb)];
+            for (int r = 0; r < R; ++r) v[q][r] = in[r * istep];
         }
     }
     __syncthreads();
+    const int ostep = lds_outR_step<COLS>(g);
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
         const int w = tid + q * g.nthr;
@@ -489,12 +573,19 @@ __device__ __forceinline__ void stage_junction(const StageGeo& g, int tid, float
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
             twiddle<R, DIR1>(v[q], tw, j);
             DftC<R, DIR1>::run(v[q]);
+            float2 f[R];
+            if (PF) {
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                v[q][r] = op.apply(v[q][r], PF ? pf[q * R + r] : op.fetch(pen, j + r * nb));
+                for (int r = 0; r < R; ++r) f[r] = pf[q * R + r];
+            } else {
+                io_fetch<R>(op, pen, j, nb, f);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = op.apply(v[q][r], f[r]);
             DftC<R, DIR2>::run(v[q]);
+            float2* o = lds + lds_outR<R, COLS>(g, w, pen, j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lidx<COLS>(g, pen, j * R + r)] = v[q][r];
+            for (int r = 0; r < R; ++r) o[r * ostep] = v[q][r];
         }
     }
     __syncthreads();
@@ -515,8 +606,7 @@ __device__ __forceinline__ void stage_prefetch(const StageGeo& g, int tid, const
         if (w < total) {
             int pen, j;
             bfly_of<COLS>(g, w, nb, nbm, pen, j);
-#pragma unroll
-            for (int r = 0; r < R; ++r) pf[q * R + r] = op.fetch(pen, j + r * nb);
+            io_fetch<R>(op, pen, j, nb, pf + q * R);
         }
     }
 }
@@ -586,10 +676,9 @@ __host__ __device__ constexpr int mb_for(int maxe, int R) { return (maxe + R - 1
 template <int MAXE, int DIR, bool COLS, int RMAX, bool SYNC, class Src>
 __device__ __forceinline__ void run_first(int R, const StageGeo& g, int tid, Src src, float2* lds)
 {
-    auto dst = [&](int pen, int i, float2 v) { lds[lidx<COLS>(g, pen, i)] = v; };
     switch (R) {
 #define CASE(RR) \
-    case RR: if constexpr (radix_ok(RMAX, RR)) { stage_first<RR, mb_for(MAXE, RR), DIR, COLS, SYNC>(g, tid, src, dst); } break;
+    case RR: if constexpr (radix_ok(RMAX, RR)) { stage_first<RR, mb_for(MAXE, RR), DIR, COLS, SYNC>(g, tid, src, lds); } break;
         DWHIP_FOR_RADICES(CASE)
 #undef CASE
     default: break;

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
         g.npen = nrows;
         g.pstride = h;
         g.nthr = XTHR;
-        g.npmagic = fdiv_magic((uint32_t)nrows);
+        g.npshift = 0;
         const uint32_t hmagic = fdiv_magic((uint32_t)h);
         const DWHIP_GL float2* __restrict__ C = gp(J.C);
         TwPtr __restrict__ rtw = gp(J.rtw);
@@ -422,15 +422,62 @@ template <bool CONJ>
 struct OtfOp {
     const DWHIP_GL float2* __restrict__ K;
     size_t stride;
-    int nb;
+    int npv;  // pencils holding data
+    static constexpr bool kStrided = true;
     __device__ __forceinline__ float2 fetch(int pen, int i) const
     {
-        return pen < nb ? ldg(K + i * stride + pen) : make_float2(0.f, 0.f);
+        return pen < npv ? ldg(K + i * stride + pen) : make_float2(0.f, 0.f);
+    }
+    // elements j + r*nb: one address per butterfly, then a uniform step
+    template <int R>
+    __device__ __forceinline__ void fetch_r(int pen, int j, int nb, float2* v) const
+    {
+        const DWHIP_GL float2* p = K + pen + (size_t)j * stride;
+        const size_t step = (size_t)nb * stride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = pen < npv ? ldg(p + r * step) : make_float2(0.f, 0.f);
     }
     __device__ __forceinline__ float2 apply(float2 v, float2 kk) const
     {
         const float a = kk.x, ac = CONJ ? -kk.y : kk.y;
         return make_float2(a * v.x - ac * v.y, a * v.y + v.x * ac);
+    }
+};
+
+// Pencils of a column tile in global memory: element i of pencil pen at
+// p[i * stride + pen].  Pencils from npv on (row padding past hc) and
+// elements from nv on (rows a pruned pass skips) read as zero and are not
+// written.
+struct ColPencils {
+    DWHIP_GL float2* p;
+    size_t stride;
+    int npv, nv;
+    static constexpr bool kStrided = true;
+    __device__ __forceinline__ float2 operator()(int pen, int i) const
+    {
+        return (pen < npv && i < nv) ? ldg(p + i * stride + pen) : make_float2(0.f, 0.f);
+    }
+    __device__ __forceinline__ void operator()(int pen, int i, float2 v) const
+    {
+        if (pen < npv && i < nv) stg(p + i * stride + pen, v);
+    }
+    template <int R>
+    __device__ __forceinline__ void load(int pen, int j, int nb, float2* v) const
+    {
+        const DWHIP_GL float2* q = p + pen + (size_t)j * stride;
+        const size_t step = (size_t)nb * stride;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = (pen < npv && j + r * nb < nv) ? ldg(q + r * step) : make_float2(0.f, 0.f);
+    }
+    template <int R>
+    __device__ __forceinline__ void store(int pen, int j, int nb, const float2* v) const
+    {
+        DWHIP_GL float2* q = p + pen + (size_t)j * stride;
+        const size_t step = (size_t)nb * stride;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (pen < npv && j + r * nb < nv) stg(q + r * step, v[r]);
     }
 };
 
@@ -486,17 +533,13 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         g.npen = B;
         g.pstride = 0;
         g.nthr = NT;
-        g.npmagic = fdiv_magic((uint32_t)B);
+        g.npshift = __ffs(B) - 1;  // B is a power of two
         // elements read (i < nin) and written (i < nout) along the pencil
         const int nin = (MODE == CM_YFWD_IMG && !J.rl) ? J.N : (MODE == CM_ZCONVJ && !J.rl) ? J.P : g.n;
         const int nout = (MODE == CM_YINV_IMG) ? J.N : (MODE == CM_ZCONV) ? J.P : g.n;
 
-        auto gsrc = [&](int pen, int i) {
-            return (pen < nb && i < nin) ? ldg(src + i * stride + pen) : make_float2(0.f, 0.f);
-        };
-        auto gdst = [&](int pen, int i, float2 v) {
-            if (pen < nb && i < nout) stg(dst + i * stride + pen, v);
-        };
+        const ColPencils gsrc{const_cast<DWHIP_GL float2*>(src), stride, nb, nin};
+        const ColPencils gdst{dst, stride, nb, nout};
         if (MODE == CM_YFWD || MODE == CM_YFWD_IMG) {
             fft_tile<DWHIP_YMAXE, -1, true, DWHIP_RMAX_Y, false, false>(J.pyf, g, tid, lds, gp(J.tw), gsrc, gdst);
         } else if (MODE == CM_YINV || MODE == CM_YINV_IMG) {
