@@ -422,7 +422,8 @@ template <bool CONJ>
 struct OtfOp {
     const DWHIP_GL float2* __restrict__ K;
     size_t stride;
-    int npv;  // pencils holding data
+    int npv;    // pencils holding data
+    bool full;  // npv is the whole tile
     static constexpr bool kStrided = true;
     __device__ __forceinline__ float2 fetch(int pen, int i) const
     {
@@ -434,8 +435,13 @@ struct OtfOp {
     {
         const DWHIP_GL float2* p = K + pen + (size_t)j * stride;
         const size_t step = (size_t)nb * stride;
+        if (full) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = pen < npv ? ldg(p + r * step) : make_float2(0.f, 0.f);
+            for (int r = 0; r < R; ++r) v[r] = ldg(p + r * step);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = pen < npv ? ldg(p + r * step) : make_float2(0.f, 0.f);
+        }
     }
     __device__ __forceinline__ float2 apply(float2 v, float2 kk) const
     {
@@ -447,11 +453,13 @@ struct OtfOp {
 // Pencils of a column tile in global memory: element i of pencil pen at
 // p[i * stride + pen].  Pencils from npv on (row padding past hc) and
 // elements from nv on (rows a pruned pass skips) read as zero and are not
-// written.
+// written; `full` (every pencil and element in range, the usual tile) skips
+// those tests.
 struct ColPencils {
     DWHIP_GL float2* p;
     size_t stride;
     int npv, nv;
+    bool full;
     static constexpr bool kStrided = true;
     __device__ __forceinline__ float2 operator()(int pen, int i) const
     {
@@ -466,18 +474,28 @@ struct ColPencils {
     {
         const DWHIP_GL float2* q = p + pen + (size_t)j * stride;
         const size_t step = (size_t)nb * stride;
+        if (full) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            v[r] = (pen < npv && j + r * nb < nv) ? ldg(q + r * step) : make_float2(0.f, 0.f);
+            for (int r = 0; r < R; ++r) v[r] = ldg(q + r * step);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                v[r] = (pen < npv && j + r * nb < nv) ? ldg(q + r * step) : make_float2(0.f, 0.f);
+        }
     }
     template <int R>
     __device__ __forceinline__ void store(int pen, int j, int nb, const float2* v) const
     {
         DWHIP_GL float2* q = p + pen + (size_t)j * stride;
         const size_t step = (size_t)nb * stride;
+        if (full) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (pen < npv && j + r * nb < nv) stg(q + r * step, v[r]);
+            for (int r = 0; r < R; ++r) stg(q + r * step, v[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (pen < npv && j + r * nb < nv) stg(q + r * step, v[r]);
+        }
     }
 };
 
@@ -538,8 +556,9 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         const int nin = (MODE == CM_YFWD_IMG && !J.rl) ? J.N : (MODE == CM_ZCONVJ && !J.rl) ? J.P : g.n;
         const int nout = (MODE == CM_YINV_IMG) ? J.N : (MODE == CM_ZCONV) ? J.P : g.n;
 
-        const ColPencils gsrc{const_cast<DWHIP_GL float2*>(src), stride, nb, nin};
-        const ColPencils gdst{dst, stride, nb, nout};
+        // (Z tiles only: the unmasked forms push 1024-thread Y tiles past 128 VGPRs)
+        const ColPencils gsrc{const_cast<DWHIP_GL float2*>(src), stride, nb, nin, !ykind && nb == B && nin == g.n};
+        const ColPencils gdst{dst, stride, nb, nout, !ykind && nb == B && nout == g.n};
         if (MODE == CM_YFWD || MODE == CM_YFWD_IMG) {
             fft_tile<DWHIP_YMAXE, -1, true, DWHIP_RMAX_Y, false, false>(J.pyf, g, tid, lds, gp(J.tw), gsrc, gdst);
         } else if (MODE == CM_YINV || MODE == CM_YINV_IMG) {
@@ -556,7 +575,7 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
             };
             fft_tile<DWHIP_ZMAXE, -1, true, DWHIP_RMAX_Z, false, false>(J.pzf, g, tid, lds, gp(J.tw), gsrc, sdst);
         } else {
-            OtfOp<MODE == CM_ZCONVJ> op{gp(J.K) + off, stride, nb};
+            OtfOp<MODE == CM_ZCONVJ> op{gp(J.K) + off, stride, nb, nb == B};
             fft_pair_tile<DWHIP_ZMAXE, -1, +1, true, DWHIP_RMAX_Z>(J.pzf, J.pzi, g, tid, lds, gp(J.tw), gsrc, op, gdst);
         }
         __syncthreads();
