@@ -654,9 +654,8 @@ struct NoOp {
 // ---------------------------------------------------------------------------
 // Radices the planner may use; each maps to a template instantiation.
 // Each axis instantiates the radices up to its own limit RMAX: x-rows up
-// to 25, y columns up to 16, z columns up to 7 (the z kernels hold the
-// OTF operand of the junction in registers too, and the smaller radices
-// keep them within 128 VGPRs).
+// to 25, y columns up to 16, z columns up to 7 (the z kernels run at
+// eight waves per SIMD, within 64 VGPRs).
 #define DWHIP_FOR_RADICES(X) \
     X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(15) X(16) X(20) X(25)
 #ifndef DWHIP_RMAX_X
@@ -713,9 +712,10 @@ __device__ __forceinline__ void run_last(int R, const StageGeo& g, int tid, cons
 
 // Junction operands held in registers between the prefetch and the
 // junction: at most DWHIP_PF_MAX per thread; junction radices needing more
-// fetch their operands at the junction.
+// fetch their operands at the junction.  Off by default: the registers
+// cost more occupancy than the early loads gain.
 #ifndef DWHIP_PF_MAX
-#define DWHIP_PF_MAX 20
+#define DWHIP_PF_MAX 0
 #endif
 __host__ __device__ constexpr bool pf_ok(int maxe, int R) { return mb_for(maxe, R) * R <= DWHIP_PF_MAX; }
 

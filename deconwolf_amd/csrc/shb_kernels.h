@@ -32,7 +32,7 @@ __host__ __device__ constexpr int cmode_kind(int m)
 #define DWHIP_CTHR 1024  // threads per workgroup, Y passes (long pencils)
 #endif
 #ifndef DWHIP_ZTHR
-#define DWHIP_ZTHR 256   // threads per workgroup, Z passes
+#define DWHIP_ZTHR 512   // threads per workgroup, Z passes
 #endif
 // dynamic LDS a workgroup may use: gfx950's 160 KiB less 1 KiB kept for
 // static LDS (the limit the kernels are raised to)
@@ -47,7 +47,7 @@ __host__ __device__ constexpr int cmode_kind(int m)
 #define DWHIP_YMAXE DWHIP_CMAXE  // Y passes
 #endif
 #ifndef DWHIP_ZMAXE
-#define DWHIP_ZMAXE DWHIP_CMAXE  // Z passes
+#define DWHIP_ZMAXE 10   // Z passes: short pencils, many waves per SIMD
 #endif
 #ifndef DWHIP_XROWS_MAX
 #define DWHIP_XROWS_MAX 64  // rows per X tile at most

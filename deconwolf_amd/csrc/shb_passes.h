@@ -517,10 +517,24 @@ constexpr int col_threads()
     return cmode_is_y(MODE) ? CTHR : ZTHR;
 }
 
-// Y tiles (1024 threads, up to 159 KiB of LDS) need 4 waves per SIMD
-// when the kernel stays within 128 VGPRs.
+// waves per SIMD the column kernels are compiled for (the VGPR budget):
+// Y tiles (1024 threads, up to 159 KiB of LDS, one per CU) need 4 and stay
+// within 128 VGPRs; Z tiles (512 threads, 10 values each, 35 KiB for
+// nz = 140) fit 8 in 64 VGPRs, four workgroups per CU
+#ifndef DWHIP_Y_WAVES
+#define DWHIP_Y_WAVES 4
+#endif
+#ifndef DWHIP_Z_WAVES
+#define DWHIP_Z_WAVES 8
+#endif
 template <int MODE>
-__global__ __launch_bounds__(col_threads<MODE>()) __attribute__((amdgpu_waves_per_eu(4)))
+constexpr int col_waves()
+{
+    return cmode_is_y(MODE) ? DWHIP_Y_WAVES : DWHIP_Z_WAVES;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(col_threads<MODE>()) __attribute__((amdgpu_waves_per_eu(col_waves<MODE>())))
 void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
 {
     extern __shared__ float2 lds[];
