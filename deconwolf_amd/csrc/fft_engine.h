@@ -305,6 +305,31 @@ __device__ __forceinline__ void stg(DWHIP_GL float2* p, float2 v)
     t.y = v.y;
     *(DWHIP_GL dwhip_f2v*)p = t;
 }
+// Streams: values a pass reads once or writes once (spectra, state, OTF).
+// NT adds the nontemporal hint; whether it pays is a property of the pass
+// (the callers choose).  Tables every tile reads again (twiddles) use ldg.
+template <bool NT>
+__device__ __forceinline__ float2 ldgs(const DWHIP_GL float2* p)
+{
+    if constexpr (NT) {
+        const dwhip_f2v t = __builtin_nontemporal_load((const DWHIP_GL dwhip_f2v*)p);
+        return make_float2(t.x, t.y);
+    } else {
+        return ldg(p);
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void stgs(DWHIP_GL float2* p, float2 v)
+{
+    if constexpr (NT) {
+        dwhip_f2v t;
+        t.x = v.x;
+        t.y = v.y;
+        __builtin_nontemporal_store(t, (DWHIP_GL dwhip_f2v*)p);
+    } else {
+        stg(p, v);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // stage geometry

@@ -111,6 +111,20 @@ struct BoolC {
 // split's scattered loads; slower for the update pass).
 __host__ __device__ constexpr bool x_stage_rows(int mode) { return DWHIP_X_STAGE_LDS && mode == XM_RATIO; }
 
+// Nontemporal hints on the passes' read-once/write-once streams.  Measured
+// on c2: the column passes and the ratio pass gain 2-10 %, the update pass
+// loses 8 % (its state streams run slower with the hint).
+#ifndef DWHIP_NT_X
+#define DWHIP_NT_X 1
+#endif
+#ifndef DWHIP_NT_UPDATE
+#define DWHIP_NT_UPDATE 0
+#endif
+#ifndef DWHIP_NT_COLS
+#define DWHIP_NT_COLS 1
+#endif
+__host__ __device__ constexpr bool x_nt(int mode) { return mode == XM_UPDATE ? DWHIP_NT_UPDATE : DWHIP_NT_X; }
+
 #ifdef DWHIP_X_WAVES
 #define DWHIP_X_OCC __attribute__((amdgpu_waves_per_eu(DWHIP_X_WAVES)))
 #else
@@ -122,6 +136,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
 {
     extern __shared__ float2 lds[];
     __shared__ double red[XTHR / 64];
+    constexpr bool XNT = x_nt(MODE);
     const int tid = threadIdx.x;
     const int total = bd->ntile[KIND_X];
     const int njobs = bd->njobs;
@@ -185,7 +200,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                     v[q] = make_float2(0.f, 0.f);
                     if (w < nraw) {
                         const int b = (int)fdiv((uint32_t)w, hcmagic);
-                        if ((live >> b) & 1ull) v[q] = ldg(C + (size_t)(row0 + b) * hcs + (w - b * hc));
+                        if ((live >> b) & 1ull) v[q] = ldgs<XNT>(C + (size_t)(row0 + b) * hcs + (w - b * hc));
                     }
                 }
 #pragma unroll
@@ -207,7 +222,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                     if (zz >= J.P || row - zz * ny >= J.N) return make_float2(0.f, 0.f);
                 }
                 const DWHIP_GL float2* crow = C + (size_t)row * hcs;
-                return c2r_split(ldg(crow + i), ldg(crow + h - i), ldg(rtw + i));
+                return c2r_split(ldgs<XNT>(crow + i), ldgs<XNT>(crow + h - i), ldg(rtw + i));
             };
             fft_tile<DWHIP_XMAXE, +1, false, DWHIP_RMAX_X, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
             __syncthreads();
@@ -250,9 +265,9 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                 if (!FULL::value && w0 + q * XTHR >= nwork) continue;
                 if (inv) a[q] = lds[w0 + q * XTHR];
                 if (MODE == XM_UPDATE) {
-                    b[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(Ukm1 + vi[q]));
-                    if (!J.rl) c[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(Uk + vi[q]));
-                    if (iter > 0 && J.has_w) wv[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(W + vi[q]));
+                    b[q] = ldgs<XNT>(reinterpret_cast<const DWHIP_GL float2*>(Ukm1 + vi[q]));
+                    if (!J.rl) c[q] = ldgs<XNT>(reinterpret_cast<const DWHIP_GL float2*>(Uk + vi[q]));
+                    if (iter > 0 && J.has_w) wv[q] = ldgs<XNT>(reinterpret_cast<const DWHIP_GL float2*>(W + vi[q]));
                 } else if (MODE == XM_LOAD) {
                     a[q] = ldg(reinterpret_cast<const DWHIP_GL float2*>(U0 + vi[q]));
                 } else if (MODE == XM_RATIO) {
@@ -309,7 +324,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                             xk.x = xk.x < bg ? bg : xk.x;
                             xk.y = xk.y < bg ? bg : xk.y;
                         }
-                        stg(reinterpret_cast<DWHIP_GL float2*>(Uk + vi[q]), xk);
+                        stgs<XNT>(reinterpret_cast<DWHIP_GL float2*>(Uk + vi[q]), xk);
                     }
                     out = xk;
                 } else if (MODE == XM_UPDATE) {
@@ -331,7 +346,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                             xk.x = xk.x < bg ? bg : xk.x;
                             xk.y = xk.y < bg ? bg : xk.y;
                         }
-                        stg(reinterpret_cast<DWHIP_GL float2*>(Uk + vi[q]), xk);
+                        stgs<XNT>(reinterpret_cast<DWHIP_GL float2*>(Uk + vi[q]), xk);
                     }
                     out = make_float2(shb_extrap(xk.x, xkm1.x, al, bg), shb_extrap(xk.y, xkm1.y, al, bg));
                 } else {  // XM_RATIO, method_shb.c:27-48 and getError dw.c:1436-1524
@@ -400,12 +415,12 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                 DWHIP_GL float2* crow = dst + (size_t)(row0 + b) * hcs;
                 if (k == 0) {
                     const float2 z0 = L[0];
-                    stg(crow, make_float2(z0.x + z0.y, 0.f));
-                    stg(crow + h, make_float2(z0.x - z0.y, 0.f));
+                    stgs<XNT>(crow, make_float2(z0.x + z0.y, 0.f));
+                    stgs<XNT>(crow + h, make_float2(z0.x - z0.y, 0.f));
                 } else {
                     const float2 a = L[k], c = L[h - k];
-                    stg(crow + k, r2c_merge(a, c, ldg(rtw + k)));
-                    if (2 * k != h) stg(crow + h - k, r2c_merge(c, a, ldg(rtw + h - k)));
+                    stgs<XNT>(crow + k, r2c_merge(a, c, ldg(rtw + k)));
+                    if (2 * k != h) stgs<XNT>(crow + h - k, r2c_merge(c, a, ldg(rtw + h - k)));
                 }
             }
         }
@@ -418,7 +433,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
 // ---------------------------------------------------------------------------
 // OTF multiply at the Z junction: fft_mul_inplace (fft.c:437-453) and
 // fft_mul_conj_inplace (fft.c:490-512) with their operand order.
-template <bool CONJ>
+template <bool CONJ, bool NT>
 struct OtfOp {
     const DWHIP_GL float2* __restrict__ K;
     size_t stride;
@@ -427,7 +442,7 @@ struct OtfOp {
     static constexpr bool kStrided = true;
     __device__ __forceinline__ float2 fetch(int pen, int i) const
     {
-        return pen < npv ? ldg(K + i * stride + pen) : make_float2(0.f, 0.f);
+        return pen < npv ? ldgs<NT>(K + i * stride + pen) : make_float2(0.f, 0.f);
     }
     // elements j + r*nb: one address per butterfly, then a uniform step
     template <int R>
@@ -437,10 +452,10 @@ struct OtfOp {
         const size_t step = (size_t)nb * stride;
         if (full) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = ldg(p + r * step);
+            for (int r = 0; r < R; ++r) v[r] = ldgs<NT>(p + r * step);
         } else {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = pen < npv ? ldg(p + r * step) : make_float2(0.f, 0.f);
+            for (int r = 0; r < R; ++r) v[r] = pen < npv ? ldgs<NT>(p + r * step) : make_float2(0.f, 0.f);
         }
     }
     __device__ __forceinline__ float2 apply(float2 v, float2 kk) const
@@ -455,6 +470,7 @@ struct OtfOp {
 // elements from nv on (rows a pruned pass skips) read as zero and are not
 // written; `full` (every pencil and element in range, the usual tile) skips
 // those tests.
+template <bool NT>
 struct ColPencils {
     DWHIP_GL float2* p;
     size_t stride;
@@ -463,11 +479,11 @@ struct ColPencils {
     static constexpr bool kStrided = true;
     __device__ __forceinline__ float2 operator()(int pen, int i) const
     {
-        return (pen < npv && i < nv) ? ldg(p + i * stride + pen) : make_float2(0.f, 0.f);
+        return (pen < npv && i < nv) ? ldgs<NT>(p + i * stride + pen) : make_float2(0.f, 0.f);
     }
     __device__ __forceinline__ void operator()(int pen, int i, float2 v) const
     {
-        if (pen < npv && i < nv) stg(p + i * stride + pen, v);
+        if (pen < npv && i < nv) stgs<NT>(p + i * stride + pen, v);
     }
     template <int R>
     __device__ __forceinline__ void load(int pen, int j, int nb, float2* v) const
@@ -476,11 +492,11 @@ struct ColPencils {
         const size_t step = (size_t)nb * stride;
         if (full) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = ldg(q + r * step);
+            for (int r = 0; r < R; ++r) v[r] = ldgs<NT>(q + r * step);
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                v[r] = (pen < npv && j + r * nb < nv) ? ldg(q + r * step) : make_float2(0.f, 0.f);
+                v[r] = (pen < npv && j + r * nb < nv) ? ldgs<NT>(q + r * step) : make_float2(0.f, 0.f);
         }
     }
     template <int R>
@@ -490,11 +506,11 @@ struct ColPencils {
         const size_t step = (size_t)nb * stride;
         if (full) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) stg(q + r * step, v[r]);
+            for (int r = 0; r < R; ++r) stgs<NT>(q + r * step, v[r]);
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                if (pen < npv && j + r * nb < nv) stg(q + r * step, v[r]);
+                if (pen < npv && j + r * nb < nv) stgs<NT>(q + r * step, v[r]);
         }
     }
 };
@@ -542,6 +558,7 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
     constexpr int kind = cmode_kind(MODE);
     constexpr bool ykind = cmode_is_y(MODE);
     constexpr int NT = col_threads<MODE>();
+    constexpr bool cnt = DWHIP_NT_COLS;
     static_assert(MODE >= 0 && MODE < DWHIP_N_CMODES, "column mode");
     const int total = bd->ntile[kind];
     const int njobs = bd->njobs;
@@ -571,8 +588,8 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         const int nout = (MODE == CM_YINV_IMG) ? J.N : (MODE == CM_ZCONV) ? J.P : g.n;
 
         // (Z tiles only: the unmasked forms push 1024-thread Y tiles past 128 VGPRs)
-        const ColPencils gsrc{const_cast<DWHIP_GL float2*>(src), stride, nb, nin, !ykind && nb == B && nin == g.n};
-        const ColPencils gdst{dst, stride, nb, nout, !ykind && nb == B && nout == g.n};
+        const ColPencils<cnt> gsrc{const_cast<DWHIP_GL float2*>(src), stride, nb, nin, !ykind && nb == B && nin == g.n};
+        const ColPencils<cnt> gdst{dst, stride, nb, nout, !ykind && nb == B && nout == g.n};
         if (MODE == CM_YFWD || MODE == CM_YFWD_IMG) {
             fft_tile<DWHIP_YMAXE, -1, true, DWHIP_RMAX_Y, false, false>(J.pyf, g, tid, lds, gp(J.tw), gsrc, gdst);
         } else if (MODE == CM_YINV || MODE == CM_YINV_IMG) {
@@ -589,7 +606,7 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
             };
             fft_tile<DWHIP_ZMAXE, -1, true, DWHIP_RMAX_Z, false, false>(J.pzf, g, tid, lds, gp(J.tw), gsrc, sdst);
         } else {
-            OtfOp<MODE == CM_ZCONVJ> op{gp(J.K) + off, stride, nb, nb == B};
+            OtfOp<MODE == CM_ZCONVJ, cnt> op{gp(J.K) + off, stride, nb, nb == B};
             fft_pair_tile<DWHIP_ZMAXE, -1, +1, true, DWHIP_RMAX_Z>(J.pzf, J.pzi, g, tid, lds, gp(J.tw), gsrc, op, gdst);
         }
         __syncthreads();
