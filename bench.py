@@ -14,9 +14,13 @@ Prints one JSON line.  `value` is image voxel-iterations per second
 (M*N*P*iterations per second of wall time over the timed steps).
 `roofline` is for the kernel with the largest total time: its algorithmic
 bytes per launch over its mean launch duration, both from HIP events
-around every launch on the stream it runs on.
+around every launch on the stream it runs on.  After the timed steps come
+the c4 request stream (`served`), the c3 tiled image (`tiled`) and a c5
+sample (`c5`: the four channels of one multi-channel field, engine against
+plain), each with its own rates; none of them is `value`.
 """
 import argparse
+import copy
 import json
 import os
 import subprocess
@@ -180,6 +184,50 @@ def run_tiled():
     return res
 
 
+def run_c5(fields=1):
+    """c5 sample: the four channels of `fields` field(s) (1024x1024x81, PSF
+    181x181x81, 100 iterations), through the engine together and one after
+    the other through the plain path; image voxel-iterations per second."""
+    import torch
+    import deconwolf_amd as dw
+    from deconwolf_amd import workloads
+    from deconwolf_amd.serve import Engine
+
+    t = time.time()
+    jobs = []
+    for f in range(fields):
+        jobs += workloads.make_c5(f)
+    torch.cuda.synchronize()
+    res = {"workload": workloads.describe_c5(fields), "input_gen_s": time.time() - t, "jobs": len(jobs)}
+    work = sum(j["im"].numel() * j["opts"].nIter for j in jobs)
+    outs = [torch.empty_like(j["im"]) for j in jobs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j, o in zip(jobs, outs):
+        dw.deconvolve_shb_hip_device(j["im"], j["psf"], copy.copy(j["opts"]), out=o)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    res["plain_value"] = work / el
+    res["plain_s"] = el
+    e = Engine(hbm_budget_gb=64.0, max_active=64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, j in enumerate(jobs):
+        e.submit(i, j["im"], j["psf"], copy.copy(j["opts"]))
+    done = set()
+    while len(done) < len(jobs):
+        done.update(e.step())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    same = all(bool(torch.equal(e.result(i)[0], outs[i])) for i in range(len(jobs)))
+    e.close()
+    res["engine_value"] = work / el
+    res["engine_s"] = el
+    res["engine_vs_plain"] = res["engine_value"] / res["plain_value"]
+    res["engine_equals_plain"] = same
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +241,7 @@ def main():
                          "engine) and plain (one at a time); 0 skips it")
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate of the stream, jobs/s")
     ap.add_argument("--no-tiled", action="store_true", help="skip the c3 tiled run")
+    ap.add_argument("--c5-fields", type=int, default=1, help="fields of the c5 sample (4 channels each); 0 skips it")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy a GIB-sized float32 buffer on the GPU 3 times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -280,6 +329,7 @@ def main():
     if a.stream > 0:
         served = run_stream(a.stream, a.rate)
     tiled = None if a.no_tiled else run_tiled()
+    c5 = run_c5(a.c5_fields) if a.c5_fields > 0 else None
 
     res = {
         "metric": METRIC,
@@ -322,6 +372,7 @@ def main():
         "cpu_baseline": cpu,
         "served": served,
         "tiled": tiled,
+        "c5": c5,
         "kernels": kernels,
         "input_gen_s": t_gen,
     }

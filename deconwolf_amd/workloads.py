@@ -175,3 +175,55 @@ def make_c3(device="cuda:0"):
     ps = psf_autocrop(psf, (P, N, M), s)
     ps = fim_normalize_sum1(ps)
     return im, ps, s
+
+
+# ---------------------------------------------------------------------------
+# c5: multi-channel fields
+# ---------------------------------------------------------------------------
+C5 = dict(shape=(81, 1024, 1024), lambdas=(460.0, 520.0, 600.0, 690.0), psf=(181, 181, 81), dz=300.0,
+          density=(500, 5000), bg=(80.0, 200.0), sigma=(1.0, 3.0), amp=(200.0, 4000.0), iters=100, fields=64)
+
+
+def describe_c5(fields):
+    return ("c5: %d of %d fields x 4 channels (460/520/600/690 nm), each 1024x1024x81 float32, Gaussian blobs "
+            "with a per-channel count U(500,5000) and background U(80,200), Born-Wolf PSF 181x181x81 (NA 1.40, "
+            "dx 65 nm, dz 300 nm), Poisson noise, 100 SHB iterations" % (fields, C5["fields"]))
+
+
+def make_c5(field, channels=(0, 1, 2, 3), device="cuda:0", iters=None):
+    """Channels of field `field` of c5: list of dicts with im (torch, offset
+    added), psf (torch, prepared like dw_run), opts, lam.  PSFs from the
+    device generator, blur with torch.fft on the GPU (input generation
+    only); seeds follow (field, channel)."""
+    import torch
+    c = C5
+    P, N, M = c["shape"]
+    pM, pN, pP = c["psf"]
+    out = []
+    for ch in channels:
+        rng = np.random.default_rng(5000 + 4 * field + ch)
+        lam = c["lambdas"][ch]
+        raw = psf_bw.bw_psf_device(pN, pP, lam, NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=c["dz"]).cpu().numpy()
+        psf = fim_normalize_sum1(raw)
+        nb = int(rng.integers(c["density"][0], c["density"][1] + 1))
+        bg = float(rng.uniform(*c["bg"]))
+        V = synth.gaussian_blobs(c["shape"], nb, sigma=c["sigma"], amp=c["amp"], rng=rng)
+        full = (P + pP - 1, N + pN - 1, M + pM - 1)
+        a = torch.as_tensor(V.astype(np.float32), device=device)
+        del V
+        b = torch.as_tensor(psf, device=device)
+        B = torch.fft.irfftn(torch.fft.rfftn(a, full) * torch.fft.rfftn(b, full), full)
+        del a
+        cz, cy, cx = (pP - 1) // 2, (pN - 1) // 2, (pM - 1) // 2
+        B = B[cz:cz + P, cy:cy + N, cx:cx + M].clamp_min(0) + bg
+        g = torch.Generator(device=device)
+        g.manual_seed(5000 + 4 * field + ch)
+        im = torch.poisson(B, generator=g).float().contiguous()
+        del B
+        s = DwOpts(nIter=iters or c["iters"], iter_type=2)
+        ps = psf_autocrop(psf, (P, N, M), s)
+        ps = fim_normalize_sum1(ps)
+        im += float(s.offset)
+        out.append(dict(im=im, psf=torch.as_tensor(ps, device=device).contiguous(), opts=s, lam=lam, bg=bg,
+                        blobs=nb))
+    return out
