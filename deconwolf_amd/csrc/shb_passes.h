@@ -575,7 +575,6 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         const int nb = min(B, J.hc - kx0);   // columns past hc are row padding, never touched
         const size_t stride = ykind ? (size_t)hcs : (size_t)hcs * ny;
         const size_t off = ykind ? (size_t)outer * ny * hcs + kx0 : (size_t)outer * hcs + kx0;
-        const DWHIP_GL float2* __restrict__ src = gp(J.C) + off;
         DWHIP_GL float2* __restrict__ dst = gp(MODE == CM_ZOTF ? J.K : J.C) + off;
         StageGeo g;
         g.n = ykind ? ny : J.nz;
@@ -588,7 +587,7 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         const int nout = (MODE == CM_YINV_IMG) ? J.N : (MODE == CM_ZCONV) ? J.P : g.n;
 
         // (Z tiles only: the unmasked forms push 1024-thread Y tiles past 128 VGPRs)
-        const ColPencils<cnt> gsrc{const_cast<DWHIP_GL float2*>(src), stride, nb, nin, !ykind && nb == B && nin == g.n};
+        const ColPencils<cnt> gsrc{gp(J.C) + off, stride, nb, nin, !ykind && nb == B && nin == g.n};
         const ColPencils<cnt> gdst{dst, stride, nb, nout, !ykind && nb == B && nout == g.n};
         if (MODE == CM_YFWD || MODE == CM_YFWD_IMG) {
             fft_tile<DWHIP_YMAXE, -1, true, DWHIP_RMAX_Y, false, false>(J.pyf, g, tid, lds, gp(J.tw), gsrc, gdst);
