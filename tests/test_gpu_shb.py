@@ -6,7 +6,7 @@ ctypes).  Tolerances are written in each test:
 * transforms: max error relative to the largest magnitude <= 2e-6;
 * SHB loop at the same transform size as the oracle's float32 run:
   relative L2 <= 1e-5 after up to 20 iterations, and <= 1e-4 against the
-  float64 oracle run (the bar of the north star for deconwolf's float SHB);
+  float64 oracle run (the bar set for deconwolf's float SHB);
 * error (I-divergence) trace: relative 1e-4 per iteration.
 """
 import numpy as np
