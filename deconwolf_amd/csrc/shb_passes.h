@@ -112,8 +112,11 @@ struct BoolC {
 __host__ __device__ constexpr bool x_stage_rows(int mode) { return DWHIP_X_STAGE_LDS && mode == XM_RATIO; }
 
 // Nontemporal hints on the passes' read-once/write-once streams.  Measured
-// on c2: the column passes and the ratio pass gain 2-10 %, the update pass
-// loses 8 % (its state streams run slower with the hint).
+// on c2: the column passes and the ratio pass gain 2-10 %.  The update pass
+// keeps plain accesses: its C2R split reads each spectrum line twice
+// (elements k and h - k, in different loads), and with the hint the line is
+// gone before the second read (x_update 8 % slower); the hint on its stores
+// alone changes nothing.
 #ifndef DWHIP_NT_X
 #define DWHIP_NT_X 1
 #endif
