@@ -68,7 +68,8 @@ class EngineConfig(ctypes.Structure):
         ("max_active", ctypes.c_int),
         ("order", ctypes.c_int),
         ("active_mvox", ctypes.c_int),
-        ("reserved", ctypes.c_int * 4),
+        ("groups", ctypes.c_int),
+        ("reserved", ctypes.c_int * 3),
     ]
 
 

@@ -117,12 +117,14 @@ struct dwhip_engine {
     dwhip_engine_config cfg;
     hipStream_t s = nullptr;
     bool own_stream = false;
+    hipStream_t s2 = nullptr;      // second group's stream (groups == 2)
+    hipEvent_t ev_setup = nullptr;  // set-up of fresh jobs done on s
     char* arena_mem = nullptr;
     Arena arena;
     std::deque<EngJob*> queue;
     std::vector<EngJob*> active;
     std::map<int64_t, EngJob*> done;
-    Table tsetup, tweights, tae, tfh;
+    Table tsetup, tweights, tae, tfh, tae2, tfh2;
     long long rounds = 0;
 };
 
@@ -297,6 +299,7 @@ int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engi
         return DWHIP_ERR_ARG;
     if ((cfg->order != DWHIP_ORDER_FIFO && cfg->order != DWHIP_ORDER_SHORTEST) || cfg->active_mvox < 0)
         return DWHIP_ERR_ARG;
+    if (cfg->groups != 0 && cfg->groups != 1 && cfg->groups != 2) return DWHIP_ERR_ARG;
     int rc = dwhip_init();
     if (rc != DWHIP_OK) return rc;
     dwhip_engine* E = new dwhip_engine();
@@ -323,6 +326,18 @@ int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engi
         dwhip_engine_destroy(E);
         return DWHIP_ERR_NOMEM;
     }
+    if (cfg->groups == 2) {
+        if (table_alloc(&E->tae2, cap) || table_alloc(&E->tfh2, cap)) {
+            dwhip_engine_destroy(E);
+            return DWHIP_ERR_NOMEM;
+        }
+        // default priority, like every stream here
+        if (hipStreamCreateWithFlags(&E->s2, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&E->ev_setup, hipEventDisableTiming) != hipSuccess) {
+            dwhip_engine_destroy(E);
+            return DWHIP_ERR_HIP;
+        }
+    }
     *out = E;
     return DWHIP_OK;
 }
@@ -331,6 +346,7 @@ void dwhip_engine_destroy(dwhip_engine* E)
 {
     if (!E) return;
     (void)hipStreamSynchronize(E->s);
+    if (E->s2) (void)hipStreamSynchronize(E->s2);
     for (EngJob* j : E->queue) delete j;
     for (EngJob* j : E->active) delete j;
     for (auto& kv : E->done) delete kv.second;
@@ -338,6 +354,10 @@ void dwhip_engine_destroy(dwhip_engine* E)
     table_free(&E->tweights);
     table_free(&E->tae);
     table_free(&E->tfh);
+    table_free(&E->tae2);
+    table_free(&E->tfh2);
+    if (E->ev_setup) (void)hipEventDestroy(E->ev_setup);
+    if (E->s2) (void)hipStreamDestroy(E->s2);
     if (E->arena_mem) (void)hipFree(E->arena_mem);
     if (E->own_stream) (void)hipStreamDestroy(E->s);
     delete E;
@@ -371,6 +391,51 @@ int dwhip_engine_submit(dwhip_engine* E, const float* im, int64_t M, int64_t N, 
     return DWHIP_OK;
 }
 
+// Splits the active jobs into two groups of about equal transform voxels
+// (largest first onto the lighter group); group membership keeps the
+// active order within each group.
+static void split_groups(const std::vector<EngJob*>& act, std::vector<EngJob*>* g)
+{
+    g[0].clear();
+    g[1].clear();
+    std::vector<size_t> idx(act.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](size_t x, size_t y) { return job_voxels(act[x]) > job_voxels(act[y]); });
+    std::vector<int> side(act.size(), 0);
+    double load[2] = {0, 0};
+    for (size_t i : idx) {
+        const int k = load[1] < load[0] ? 1 : 0;
+        side[i] = k;
+        load[k] += job_voxels(act[i]);
+    }
+    for (size_t i = 0; i < act.size(); ++i) g[side[i]].push_back(act[i]);
+}
+
+// One iteration of every job of a group on stream s; the adjoint half
+// (F, G, H) only for jobs with a further iteration to feed.
+static int group_round(Table* tae, Table* tfh, const std::vector<EngJob*>& jobs, hipStream_t s)
+{
+    if (jobs.empty()) return DWHIP_OK;
+    std::vector<EngJob*> fh;
+    for (EngJob* j : jobs) {
+        const bool fixed = j->prm.iter_type == DWHIP_ITER_FIXED;
+        if (!fixed || j->j.d.rl || j->j.host_iter + 1 < j->prm.niter) fh.push_back(j);
+    }
+    RCHK(table_build(tae, jobs, s));
+    RCHK(table_build(tfh, fh, s));
+    RCHK(table_pass(*tae, true, XM_UPDATE, s));
+    RCHK(table_pass(*tae, false, CM_YFWD, s));
+    RCHK(table_pass(*tae, false, CM_ZCONV, s));
+    RCHK(table_pass(*tae, false, CM_YINV_IMG, s));
+    RCHK(table_pass(*tae, true, XM_RATIO, s));
+    RCHK(table_pass(*tfh, false, CM_YFWD_IMG, s));
+    RCHK(table_pass(*tfh, false, CM_ZCONVJ, s));
+    RCHK(table_pass(*tfh, false, CM_YINV, s));
+    ECHK(dwhip_launch_round_end(tae->bd(), tae->jd(), tae->n, 1, s));
+    return DWHIP_OK;
+}
+
 int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_done)
 {
     if (!E) return DWHIP_ERR_ARG;
@@ -383,24 +448,23 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         ECHK(hipStreamSynchronize(s));
         return DWHIP_OK;
     }
-    // one iteration of every active job; the adjoint half (F, G, H) only
-    // for jobs with a further iteration to feed
-    std::vector<EngJob*> fh;
-    for (EngJob* j : E->active) {
-        const bool fixed = j->prm.iter_type == DWHIP_ITER_FIXED;
-        if (!fixed || j->j.d.rl || j->j.host_iter + 1 < j->prm.niter) fh.push_back(j);
+    // the jobs of this round, in one group on s or two on s and s2
+    std::vector<EngJob*> grp[2];
+    const bool two = E->s2 && E->active.size() >= 2;
+    if (two) {
+        split_groups(E->active, grp);
+        // the second group's jobs may have been set up on s in this step
+        ECHK(hipEventRecord(E->ev_setup, s));
+        ECHK(hipStreamWaitEvent(E->s2, E->ev_setup, 0));
+    } else {
+        grp[0] = E->active;
     }
-    RCHK(table_build(&E->tae, E->active, s));
-    RCHK(table_build(&E->tfh, fh, s));
-    RCHK(table_pass(E->tae, true, XM_UPDATE, s));
-    RCHK(table_pass(E->tae, false, CM_YFWD, s));
-    RCHK(table_pass(E->tae, false, CM_ZCONV, s));
-    RCHK(table_pass(E->tae, false, CM_YINV_IMG, s));
-    RCHK(table_pass(E->tae, true, XM_RATIO, s));
-    RCHK(table_pass(E->tfh, false, CM_YFWD_IMG, s));
-    RCHK(table_pass(E->tfh, false, CM_ZCONVJ, s));
-    RCHK(table_pass(E->tfh, false, CM_YINV, s));
-    ECHK(dwhip_launch_round_end(E->tae.bd(), E->tae.jd(), E->tae.n, 1, s));
+    hipStream_t gs[2] = {s, two ? E->s2 : s};
+    RCHK(group_round(&E->tae, &E->tfh, grp[0], gs[0]));
+    if (two) RCHK(group_round(&E->tae2, &E->tfh2, grp[1], gs[1]));
+    std::map<EngJob*, hipStream_t> js;  // each job's stream this round
+    for (int g = 0; g < 2; ++g)
+        for (EngJob* j : grp[g]) js[j] = gs[g];
     for (EngJob* j : E->active) {
         j->j.host_iter += 1;
         j->nrun = j->j.host_iter;
@@ -412,11 +476,14 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
     for (size_t i = 0; i < E->active.size(); ++i) {
         EngJob* j = E->active[i];
         if (j->prm.iter_type != DWHIP_ITER_FIXED) {
-            ECHK(hipMemcpyAsync(&errs[i], j->j.d.err + (j->nrun - 1), sizeof(float), hipMemcpyDeviceToHost, s));
+            ECHK(hipMemcpyAsync(&errs[i], j->j.d.err + (j->nrun - 1), sizeof(float), hipMemcpyDeviceToHost, js[j]));
             need_err = true;
         }
     }
-    if (need_err) ECHK(hipStreamSynchronize(s));
+    if (need_err) {
+        ECHK(hipStreamSynchronize(s));
+        if (two) ECHK(hipStreamSynchronize(E->s2));
+    }
     // finished jobs: crop the result into out
     std::vector<EngJob*> finished, still;
     for (size_t i = 0; i < E->active.size(); ++i) {
@@ -431,14 +498,15 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         }
         if (fin) {
             // SHB returns x_{n-1}; RL returns x_n, after one more update pass
-            if (j->j.d.rl) RCHK(dwhip_job_final_update(&j->j, j->j.bd, j->j.jd, s));
-            ECHK(dwhip_launch_crop(j->j.jd, 0, j->j.d.rl ? j->nrun : j->nrun - 1, j->out, 1024, s));
+            if (j->j.d.rl) RCHK(dwhip_job_final_update(&j->j, j->j.bd, j->j.jd, js[j]));
+            ECHK(dwhip_launch_crop(j->j.jd, 0, j->j.d.rl ? j->nrun : j->nrun - 1, j->out, 1024, js[j]));
             finished.push_back(j);
         } else {
             still.push_back(j);
         }
     }
     ECHK(hipStreamSynchronize(s));
+    if (two) ECHK(hipStreamSynchronize(E->s2));
     if (dwhip_prof_on()) dwhip_prof_collect();
     for (EngJob* j : finished) {
         j->err.resize(j->nrun);

@@ -22,9 +22,10 @@ from .method_shb import DwOpts, _params
 class Engine:
     """order: _lib.ORDER_FIFO (arrival order) or ORDER_SHORTEST (least work
     first); active_mvox: cap on the transform voxels (millions) of the jobs
-    iterating together, 0 for none."""
+    iterating together, 0 for none; groups: 2 runs the active jobs as two
+    groups on two streams whose passes overlap."""
 
-    def __init__(self, hbm_budget_gb=32.0, max_active=64, stream=None, order=0, active_mvox=0):
+    def __init__(self, hbm_budget_gb=32.0, max_active=64, stream=None, order=0, active_mvox=0, groups=1):
         import torch
         cfg = EngineConfig()
         lib().dwhip_engine_config_default(ctypes.byref(cfg))
@@ -32,6 +33,7 @@ class Engine:
         cfg.max_active = int(max_active)
         cfg.order = int(order)
         cfg.active_mvox = int(active_mvox)
+        cfg.groups = int(groups)
         h = ctypes.c_void_p()
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().dwhip_engine_create(ctypes.byref(cfg), s, ctypes.byref(h)), "dwhip_engine_create")

@@ -239,7 +239,10 @@ typedef struct {
     int order;          /* DWHIP_ORDER_*; default FIFO */
     int active_mvox;    /* cap on the transform voxels (millions) of the jobs iterating
                            together, 0 = none; a job larger than the cap runs alone */
-    int reserved[4];
+    int groups;         /* 1 (default) or 2: the active jobs split in two groups of about
+                           equal voxels whose rounds run on two streams, so that passes
+                           of one group overlap those of the other */
+    int reserved[3];
 } dwhip_engine_config;
 
 void dwhip_engine_config_default(dwhip_engine_config *cfg);

@@ -1,6 +1,7 @@
 """GPU tests of the batched/served engine (dwhip_engine_*): every job's
 result in a mixed batch is bit-identical to the same job run alone through
-dwhip_shb_run_device, whatever runs beside it, however it is admitted."""
+dwhip_shb_run_device, whatever runs beside it, however it is admitted, with
+the active jobs in one group or in two groups on two streams."""
 import numpy as np
 import pytest
 
@@ -45,10 +46,11 @@ def solo(im, k, s):
     return out.cpu().numpy(), s2
 
 
-def test_batch_equals_alone():
-    print("engine: mixed batch of", len(JOBS), flush=True)
+@pytest.mark.parametrize("groups", [1, 2])
+def test_batch_equals_alone(groups):
+    print("engine: mixed batch of", len(JOBS), "groups", groups, flush=True)
     jobs = [make_job(sd, sh, ps, n, **o) for sd, sh, ps, n, o in JOBS]
-    eng = Engine(hbm_budget_gb=2.0, max_active=16)
+    eng = Engine(hbm_budget_gb=2.0, max_active=16, groups=groups)
     for t, (im, k, s) in enumerate(jobs):
         eng.submit(t, im, k, s)
     done = []
@@ -68,13 +70,14 @@ def test_batch_equals_alone():
     eng.close()
 
 
-def test_budget_limited_admission_and_staggered_submission():
-    print("engine: budget-limited admission", flush=True)
+@pytest.mark.parametrize("groups", [1, 2])
+def test_budget_limited_admission_and_staggered_submission(groups):
+    print("engine: budget-limited admission, groups", groups, flush=True)
     jobs = [make_job(10 + i, (16, 36, 40), (7, 9, 9), 3 + i) for i in range(5)]
     w, r, _ = dw.work_size((16, 36, 40), (7, 9, 9))
     V = w[0] * w[1] * w[2]
     # room for about two of these jobs (a job is ~5 V floats)
-    eng = Engine(hbm_budget_gb=2.5 * 5 * V * 4 / (1 << 30), max_active=4)
+    eng = Engine(hbm_budget_gb=2.5 * 5 * V * 4 / (1 << 30), max_active=4, groups=groups)
     for t in range(2):
         eng.submit(t, *jobs[t])
     seen_active = 0
@@ -100,13 +103,15 @@ def test_budget_limited_admission_and_staggered_submission():
     eng.close()
 
 
-def test_rel_stopping_in_engine():
+@pytest.mark.parametrize("groups", [1, 2])
+def test_rel_stopping_in_engine(groups):
+    print("engine: REL stopping, groups", groups, flush=True)
     im, k, s = make_job(21, (10, 24, 30), (5, 7, 7), 1)
     s.iter_type = dw.ITER_REL
     s.err_rel = 0.05
     s.maxiter = 40
     ref, s2 = solo(im, k, s)
-    eng = Engine(hbm_budget_gb=0.5, max_active=2)
+    eng = Engine(hbm_budget_gb=0.5, max_active=2, groups=groups)
     eng.submit(0, im, k, s)
     im2, k2, s3 = make_job(22, (16, 36, 40), (7, 9, 9), 30)
     eng.submit(1, im2, k2, s3)

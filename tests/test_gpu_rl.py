@@ -97,9 +97,11 @@ def test_rl_rel_stopping_and_mse():
     np.testing.assert_allclose(s.err_trace, e32, rtol=1e-4)
 
 
-def test_rl_device_entry_and_engine_bit_identical():
+@pytest.mark.parametrize("groups", [1, 2])
+def test_rl_device_entry_and_engine_bit_identical(groups):
     """Host entry, device entry and the batched engine (RL and SHB jobs in
-    one batch) give the same bytes."""
+    one batch, in one group or in two groups on two streams) give the same
+    bytes."""
     from deconwolf_amd.serve import Engine
     im, k = case((12, 30, 34), (5, 7, 7), seed=27)
     im2, k2 = case((9, 22, 26), (5, 5, 7), seed=28)
@@ -108,7 +110,7 @@ def test_rl_device_entry_and_engine_bit_identical():
     torch.cuda.synchronize()
     assert np.array_equal(a, b.cpu().numpy())
     shb = dw.deconvolve_shb_hip(im2, k2, dw.DwOpts(nIter=5, iter_type=dw.ITER_FIXED))
-    eng = Engine(hbm_budget_gb=1.0, max_active=8)
+    eng = Engine(hbm_budget_gb=1.0, max_active=8, groups=groups)
     eng.submit(1, torch.from_numpy(im).to(DEV), torch.from_numpy(k).to(DEV), _opts(7))
     eng.submit(2, torch.from_numpy(im2).to(DEV), torch.from_numpy(k2).to(DEV),
                dw.DwOpts(nIter=5, iter_type=dw.ITER_FIXED))
