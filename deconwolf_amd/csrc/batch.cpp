@@ -291,6 +291,7 @@ void dwhip_engine_config_default(dwhip_engine_config* c)
     memset(c, 0, sizeof(*c));
     c->hbm_budget = (size_t)32 << 30;
     c->max_active = 64;
+    c->groups = 1;
 }
 
 int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engine** out)
@@ -299,7 +300,7 @@ int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engi
         return DWHIP_ERR_ARG;
     if ((cfg->order != DWHIP_ORDER_FIFO && cfg->order != DWHIP_ORDER_SHORTEST) || cfg->active_mvox < 0)
         return DWHIP_ERR_ARG;
-    if (cfg->groups != 0 && cfg->groups != 1 && cfg->groups != 2) return DWHIP_ERR_ARG;
+    if (cfg->groups != 0 && cfg->groups != 1 && cfg->groups != 2) return DWHIP_ERR_ARG;  // 0 reads as 1
     int rc = dwhip_init();
     if (rc != DWHIP_OK) return rc;
     dwhip_engine* E = new dwhip_engine();

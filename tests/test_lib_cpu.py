@@ -116,6 +116,23 @@ def test_no_gpu_fails_loudly():
     assert np.array_equal(out, im)
 
 
+def test_engine_config_validated_before_device_use():
+    """dwhip_engine_create checks its configuration before touching a
+    device: bad values give DWHIP_ERR_ARG on any host; the defaults are
+    FIFO admission, no voxel cap, one job group, 64 active jobs."""
+    L = _lib.lib()
+    cfg = _lib.EngineConfig()
+    L.dwhip_engine_config_default(ctypes.byref(cfg))
+    assert (cfg.max_active, cfg.order, cfg.active_mvox, cfg.groups) == (64, _lib.ORDER_FIFO, 0, 1)
+    assert ctypes.sizeof(_lib.EngineConfig) == 40   # hbm_budget + 7 ints, 8-byte aligned, as in dwhip.h
+    h = ctypes.c_void_p()
+    for field, bad in (("groups", 3), ("groups", -1), ("order", 7), ("active_mvox", -5), ("max_active", 0)):
+        c = _lib.EngineConfig()
+        L.dwhip_engine_config_default(ctypes.byref(c))
+        setattr(c, field, bad)
+        assert _lib.ERRORS.get(L.dwhip_engine_create(ctypes.byref(c), None, ctypes.byref(h))) == "DWHIP_ERR_ARG", field
+
+
 class View(ctypes.Structure):
     _fields_ = [(n, t) for n, t in [
         ("nIter", ctypes.c_int), ("iter_type", ctypes.c_int), ("maxiter", ctypes.c_int),
