@@ -176,7 +176,7 @@ int table_build(Table* t, const std::vector<EngJob*>& jobs, hipStream_t s)
 // One pass over a table, with the profiling hooks of the per-image path.
 int table_pass(const Table& t, bool xkind, int mode, hipStream_t s)
 {
-    const int kind = xkind ? KIND_X : cmode_kind(mode);
+    const int kind = xkind ? xmode_kind(mode) : cmode_kind(mode);
     if (t.n == 0 || t.ntile[kind] == 0) return DWHIP_OK;
     const int slot = dwhip_prof_begin(s);
     hipError_t e = xkind ? dwhip_launch_xpass(mode, t.bd(), t.jd(), t.ntile[kind], t.lds[kind], s)

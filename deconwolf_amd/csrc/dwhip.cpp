@@ -245,6 +245,12 @@ static int pow2_floor(int v)
     return p;
 }
 
+// x tiles of pass E for SHB: ceil(N / xrows) tiles in each of the P planes
+static int ratio_tiles(int64_t N, int64_t P, int xrows)
+{
+    return (int)(((N + xrows - 1) / xrows) * P);
+}
+
 // Tile shapes and LDS bytes for a transform size.
 void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
 {
@@ -267,6 +273,8 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
     // the pruned y passes: all slabs until the job's image depth is known
     t->ntile[KIND_YD] = t->ntile[KIND_YF] = t->ntile[KIND_Y];
     t->lds[KIND_YD] = t->lds[KIND_YF] = t->lds[KIND_Y];
+    t->ntile[KIND_XR] = t->ntile[KIND_X];
+    t->lds[KIND_XR] = t->lds[KIND_X];
 }
 
 static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -281,7 +289,7 @@ size_t dwhip_job_bytes(const int64_t* work, int64_t M, int64_t N, int64_t P, int
     dwhip_tile_config(work, &t);
     size_t b = 2 * Vc * sizeof(float2);                  // C, K
     b += (border_quality > 0 ? 3 : 2) * V * sizeof(float); // U0, U1, W
-    b += (size_t)t.ntile[KIND_X] * sizeof(double);
+    b += (size_t)std::max(t.ntile[KIND_X], ratio_tiles(N, P, t.xrows)) * sizeof(double);
     b += (size_t)(niter + 1) * (sizeof(float) + sizeof(double));
     b += DWHIP_STAT_BLOCKS * 4 * sizeof(double) + 4096;
     (void)M; (void)N; (void)P; (void)pM; (void)pN; (void)pP;
@@ -353,6 +361,9 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
         const int nct = (d.hc + j->tiles.ycols - 1) / j->tiles.ycols;
         j->tiles.ntile[KIND_YD] = nct * (int)P;
         j->tiles.ntile[KIND_YF] = prm->method == DWHIP_METHOD_RL ? nct * (int)w[2] : nct * (int)P;
+        // pass E: rows y < N of planes z < P for SHB (whose ratio is 0
+        // elsewhere), all rows for RL
+        if (prm->method != DWHIP_METHOD_RL) j->tiles.ntile[KIND_XR] = ratio_tiles(N, P, j->tiles.xrows);
     }
     for (int k = 0; k < KIND_N; ++k) d.ntile[k] = j->tiles.ntile[k];
     d.xrows = j->tiles.xrows;
@@ -392,7 +403,7 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     L.U0 = take(V * sizeof(float));
     L.U1 = take(V * sizeof(float));
     L.W = d.has_w ? take(V * sizeof(float)) : 0;
-    L.part = take((size_t)d.ntile[KIND_X] * sizeof(double));
+    L.part = take((size_t)std::max(d.ntile[KIND_X], d.ntile[KIND_XR]) * sizeof(double));
     L.stat = take(DWHIP_STAT_BLOCKS * 4 * sizeof(double));
     L.err = take((size_t)niter * sizeof(float));
     L.errd = take((size_t)niter * sizeof(double));
@@ -556,7 +567,7 @@ int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s)
     RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
     RCHK(dwhip_pass_c(CM_ZCONV, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));
     RCHK(dwhip_pass_c(CM_YINV_IMG, j, j->bd, j->jd, t.ntile[KIND_YD], t.lds[KIND_YD], s));
-    RCHK(dwhip_pass_x(XM_RATIO, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
+    RCHK(dwhip_pass_x(XM_RATIO, j, j->bd, j->jd, t.ntile[KIND_XR], t.lds[KIND_XR], s));
     if (full) {
         RCHK(dwhip_pass_c(CM_YFWD_IMG, j, j->bd, j->jd, t.ntile[KIND_YF], t.lds[KIND_YF], s));
         RCHK(dwhip_pass_c(CM_ZCONVJ, j, j->bd, j->jd, t.ntile[KIND_Z], t.lds[KIND_Z], s));

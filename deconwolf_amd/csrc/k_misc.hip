@@ -129,7 +129,7 @@ __global__ __launch_bounds__(STHR) void k_round_end(const BatchDev* __restrict__
 {
     __shared__ double red[STHR / 64];
     const JobDev& J = jobs[blockIdx.x];
-    const int nt = J.ntile[KIND_X];
+    const int nt = J.ntile[KIND_XR];  // one partial per tile of pass E
     // each thread sums its strided partials in index order; the loads of
     // eight of them are issued together
     const double* __restrict__ part = J.part;

@@ -11,11 +11,13 @@
 #include "fft_engine.h"
 
 // Tile spaces: x-row tiles, y column tiles of all slabs, z column tiles,
-// and the y tiles of the pruned passes — KIND_YD: slabs z < P (pass D);
-// KIND_YF: pass F's slabs (z < P for SHB, all of them for RL).  A pruned
-// pass enumerates only the tiles it works on, so its grid carries no idle
-// workgroups and every XCD gets a share of the work.
-enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2, KIND_YD = 3, KIND_YF = 4, KIND_N = 5 };
+// and the tiles of the pruned passes — KIND_YD: slabs z < P (pass D);
+// KIND_YF: pass F's slabs (z < P for SHB, all of them for RL); KIND_XR:
+// pass E's rows (for SHB the rows y < N of the planes z < P, in tiles that
+// stay within one plane; for RL all x tiles).  A pruned pass enumerates only
+// the tiles it works on, so its grid carries no idle workgroups and every
+// XCD gets a share of the work.
+enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2, KIND_YD = 3, KIND_YF = 4, KIND_XR = 5, KIND_N = 6 };
 
 // metric / start values mirror dw_metric and dw_start_condition (src/dw.h:64-77)
 enum { DWHIP_DEV_METRIC_MSE = 0, DWHIP_DEV_METRIC_IDIV = 1 };

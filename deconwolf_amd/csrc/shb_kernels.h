@@ -19,6 +19,8 @@ __host__ __device__ constexpr bool cmode_is_y(int m)
 {
     return m == CM_YFWD || m == CM_YINV || m == CM_YINV_IMG || m == CM_YFWD_IMG;
 }
+// tile space of an x mode
+__host__ __device__ constexpr int xmode_kind(int m) { return m == XM_RATIO ? KIND_XR : KIND_X; }
 // tile space of a column mode
 __host__ __device__ constexpr int cmode_kind(int m)
 {

@@ -141,27 +141,27 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
     __shared__ double red[XTHR / 64];
     constexpr bool XNT = x_nt(MODE);
     const int tid = threadIdx.x;
-    const int total = bd->ntile[KIND_X];
+    constexpr int kind = xmode_kind(MODE);
+    const int total = bd->ntile[kind];
     const int njobs = bd->njobs;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const JobDev& J = jobs[find_job(jobs, njobs, KIND_X, t)];
-        const int lt = t - J.tile0[KIND_X];
+        const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
+        const int lt = t - J.tile0[kind];
         const int h = J.h, nx = J.nx, ny = J.ny, hcs = J.hcs;
-        const int row0 = lt * J.xrows;
-        const int nrows = min(J.xrows, ny * J.nz - row0);
+        int row0 = lt * J.xrows;
+        int nrows = min(J.xrows, ny * J.nz - row0);
+        if (MODE == XM_RATIO && !J.rl) {
+            // the SHB ratio is 0 outside the image box and only y < N, z < P
+            // rows of y are used: pass E's tiles cover exactly those rows,
+            // ceil(N / xrows) tiles within each plane z < P (the RL ratio is
+            // 1e-6 outside the image, so RL runs the plain x tiles)
+            const int tpp = (J.N + J.xrows - 1) / J.xrows;
+            const int z = lt / tpp, k = lt - z * tpp;
+            row0 = z * ny + k * J.xrows;
+            nrows = min(J.xrows, J.N - k * J.xrows);
+        }
         const DWHIP_GL JobState* __restrict__ st = gp(J.st);
         const int iter = (MODE == XM_UPDATE || MODE == XM_RATIO) ? st->iter : 0;
-        // the SHB ratio is 0 outside the image box and only y < N, z < P rows
-        // of y are used: pass E reads and writes only those rows (the RL
-        // ratio is 1e-6 outside the image, so RL writes every row)
-        if (MODE == XM_RATIO && !J.rl) {
-            const int z0 = row0 / ny, y0 = row0 - z0 * ny;
-            const int zl = (row0 + nrows - 1) / ny;
-            if (z0 >= J.P || (z0 == zl && y0 >= J.N)) {
-                if (tid == 0) gp(J.part)[lt] = 0.0;
-                continue;
-            }
-        }
         const bool inv = MODE == XM_RATIO || MODE == XM_WEIGHTS || MODE == XM_STORE ||
                          (MODE == XM_UPDATE && iter > 0);
         const bool fwd = MODE != XM_WEIGHTS && MODE != XM_STORE;
