@@ -22,8 +22,8 @@ START_FLAT, START_LP, START_IDENTITY = 0, 1, 2
 U16_ROUND, U16_TRUNC, U16_RINT = 0, 1, 2
 # dwhip_engine_config.order
 ORDER_FIFO, ORDER_SHORTEST = 0, 1
-# dwhip_params.method
-DWHIP_METHOD_SHB, DWHIP_METHOD_RL = 0, 1
+# dwhip_params.method: the dw_method values (src/dw.h:52-60)
+DWHIP_METHOD_RL, DWHIP_METHOD_SHB = 0, 2
 # dw_method (src/dw.h:52-60), the values of DwOpts.method
 METHOD_RL, METHOD_ID, METHOD_SHB = 0, 1, 2
 

@@ -13,10 +13,10 @@
  *     (method_shb.c:108-117);
  *   - the log gets the job size and one line per iteration, like
  *     deconvolve_shb and dw_iterator_show (src/dw.c:321-333).
- * Errors: a transform size libdwhip cannot run (border quality 0 or 1 with
- * sizes that do not factor into 2, 3, 5, 7) falls back to deconwolf's CPU
- * deconvolve_shb, with a note in the log; any other error prints the
- * reason and exits with EXIT_FAILURE, as deconwolf does on fatal errors.
+ * Errors: any libdwhip error, a transform size it cannot run included,
+ * prints the reason (and the job size) and exits with EXIT_FAILURE, as
+ * deconwolf does on fatal errors (method_shb.c:199-203).  Nothing here
+ * calls deconwolf's own CPU methods.
  */
 #include <inttypes.h>
 #include <stdio.h>
@@ -24,8 +24,6 @@
 #include <string.h>
 
 #include "method_shb_hip.h"
-#include "method_shb.h"
-#include "method_rl.h"
 #include "fim.h"
 #include "dwhip.h"
 #include "dwhip_adapter.h"
@@ -59,12 +57,14 @@ static float *run_hip(int rl, float *restrict im, const int64_t M, const int64_t
     dwhip_adapter_params(&v, &p);
 
     int64_t work[3], ref[3];
-    if (dwhip_work_size(M, N, P, pM, pN, pP, s->borderQuality, work, ref) == DWHIP_ERR_SIZE) {
-        fprintf(s->log, "%s: job [%" PRId64 "x%" PRId64 "x%" PRId64 "] not supported on the GPU, "
-                "using the CPU method\n", tag, ref[0], ref[1], ref[2]);
-        if (rl)
-            return deconvolve_rl(im, M, N, P, psf, pM, pN, pP, s); /* frees psf */
-        return deconvolve_shb(im, M, N, P, psf, pM, pN, pP, s); /* frees psf */
+    const int wrc = dwhip_work_size(M, N, P, pM, pN, pP, s->borderQuality, work, ref);
+    if (wrc != DWHIP_OK) {
+        fprintf(stderr, "%s: job [%" PRId64 "x%" PRId64 "x%" PRId64 "] (border quality %d): %s\n", tag,
+                ref[0], ref[1], ref[2], s->borderQuality, dwhip_strerror(wrc));
+        if (s->log != NULL && s->log != stdout)
+            fprintf(s->log, "%s: job [%" PRId64 "x%" PRId64 "x%" PRId64 "]: %s\n", tag, ref[0], ref[1], ref[2],
+                    dwhip_strerror(wrc));
+        exit(EXIT_FAILURE);
     }
 
     float *im_lp = NULL;

@@ -392,6 +392,26 @@ int dwhip_engine_submit(dwhip_engine* E, const float* im, int64_t M, int64_t N, 
     return DWHIP_OK;
 }
 
+// Completed jobs (their crops queued and the streams synchronised): error
+// trace and bg read back, memory returned to the arena, tags reported.
+static int finish_jobs(dwhip_engine* E, const std::vector<EngJob*>& finished, int64_t* done_tags, int max_done,
+                       int* n_done)
+{
+    for (EngJob* j : finished) {
+        j->err.resize(j->nrun);
+        if (j->nrun > 0)
+            ECHK(hipMemcpy(j->err.data(), j->j.d.err, j->nrun * sizeof(float), hipMemcpyDeviceToHost));
+        JobState hs;
+        ECHK(hipMemcpy(&hs, j->j.d.st, sizeof(hs), hipMemcpyDeviceToHost));
+        j->bg = hs.bg;
+        E->arena.release(j->off, j->j.bytes);
+        j->j.mem = nullptr;
+        E->done[j->tag] = j;
+        if (done_tags && n_done && *n_done < max_done) done_tags[(*n_done)++] = j->tag;
+    }
+    return DWHIP_OK;
+}
+
 // Splits the active jobs into two groups of about equal transform voxels
 // (largest first onto the lighter group); group membership keeps the
 // active order within each group.
@@ -445,9 +465,24 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
     std::vector<EngJob*> fresh;
     RCHK(admit(E, fresh));
     RCHK(setup(E, fresh));
+    // REL/ABS jobs whose stopping rule ends them before iteration 0 (REL with
+    // err_rel in (1/3, 1)): no round, the start guess is their result
+    std::vector<EngJob*> finished;
+    {
+        std::vector<EngJob*> run;
+        for (EngJob* j : E->active) {
+            if (j->nrun == 0 && j->prm.iter_type != DWHIP_ITER_FIXED && !keep_going(j, 0)) {
+                ECHK(dwhip_launch_crop(j->j.jd, 0, dwhip_result_iterate(j->j.d.rl != 0, 0), j->out, 1024, s));
+                finished.push_back(j);
+            } else {
+                run.push_back(j);
+            }
+        }
+        E->active = run;
+    }
     if (E->active.empty()) {
         ECHK(hipStreamSynchronize(s));
-        return DWHIP_OK;
+        return finish_jobs(E, finished, done_tags, max_done, n_done);
     }
     // the jobs of this round, in one group on s or two on s and s2
     std::vector<EngJob*> grp[2];
@@ -486,7 +521,7 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         if (two) ECHK(hipStreamSynchronize(E->s2));
     }
     // finished jobs: crop the result into out
-    std::vector<EngJob*> finished, still;
+    std::vector<EngJob*> still;
     for (size_t i = 0; i < E->active.size(); ++i) {
         EngJob* j = E->active[i];
         bool fin;
@@ -500,7 +535,7 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         if (fin) {
             // SHB returns x_{n-1}; RL returns x_n, after one more update pass
             if (j->j.d.rl) RCHK(dwhip_job_final_update(&j->j, j->j.bd, j->j.jd, js[j]));
-            ECHK(dwhip_launch_crop(j->j.jd, 0, j->j.d.rl ? j->nrun : j->nrun - 1, j->out, 1024, js[j]));
+            ECHK(dwhip_launch_crop(j->j.jd, 0, dwhip_result_iterate(j->j.d.rl != 0, j->nrun), j->out, 1024, js[j]));
             finished.push_back(j);
         } else {
             still.push_back(j);
@@ -509,19 +544,8 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
     ECHK(hipStreamSynchronize(s));
     if (two) ECHK(hipStreamSynchronize(E->s2));
     if (dwhip_prof_on()) dwhip_prof_collect();
-    for (EngJob* j : finished) {
-        j->err.resize(j->nrun);
-        ECHK(hipMemcpy(j->err.data(), j->j.d.err, j->nrun * sizeof(float), hipMemcpyDeviceToHost));
-        JobState hs;
-        ECHK(hipMemcpy(&hs, j->j.d.st, sizeof(hs), hipMemcpyDeviceToHost));
-        j->bg = hs.bg;
-        E->arena.release(j->off, j->j.bytes);
-        j->j.mem = nullptr;
-        E->done[j->tag] = j;
-        if (done_tags && n_done && *n_done < max_done) done_tags[(*n_done)++] = j->tag;
-    }
     E->active = still;
-    return DWHIP_OK;
+    return finish_jobs(E, finished, done_tags, max_done, n_done);
 }
 
 int dwhip_engine_status(const dwhip_engine* E, int* queued, int* active, size_t* bytes_in_use)

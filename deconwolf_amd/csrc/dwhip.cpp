@@ -648,8 +648,11 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
     // RL returns x_n: one more update pass turns the last adjoint into x_n
     if (rc == DWHIP_OK && nrun > 0 && job.d.rl) rc = dwhip_job_final_update(&job, job.bd, job.jd, s);
     hipEventRecord(e2, s);
-    if (rc == DWHIP_OK && nrun > 0) {
-        hipError_t e = dwhip_launch_crop(job.jd, 0, job.d.rl ? nrun : nrun - 1, out, 1024, s);
+    if (rc == DWHIP_OK) {
+        // with no iteration run (REL stopping before iteration 0) the result
+        // is the start guess x_0, which the swap-back of method_shb.c:385-390
+        // hands back (deconvolve_rl would crop a NULL iterate there)
+        hipError_t e = dwhip_launch_crop(job.jd, 0, dwhip_result_iterate(job.d.rl != 0, nrun), out, 1024, s);
         if (e != hipSuccess) rc = dwhip_hip_fail(e, "crop", __LINE__);
     }
     hipError_t e = hipStreamSynchronize(s);

@@ -44,6 +44,14 @@ struct DwhipJob {
 
 int dwhip_hip_fail(hipError_t e, const char* what, int line);
 
+// The iterate a method returns after nrun iterations: SHB x_{n-1} (swap-back,
+// method_shb.c:385-390), RL x_n; the start guess x_0 when nothing ran.
+inline int dwhip_result_iterate(bool rl, int nrun)
+{
+    if (nrun <= 0) return 0;
+    return rl ? nrun : nrun - 1;
+}
+
 // profiling of pass launches (profile.cpp); kernel ids: X modes 0..6, then
 // column modes 7..15
 #define DWHIP_PROF_KERNELS 16

@@ -79,12 +79,10 @@ def _params(s: DwOpts) -> DwhipParams:
     for i in range(3):
         p.work[i] = int(w[i])
     p.verbosity = int(s.verbosity)
-    if int(s.method) == METHOD_SHB:
-        p.method = DWHIP_METHOD_SHB
-    elif int(s.method) == METHOD_RL:
-        p.method = DWHIP_METHOD_RL
-    else:
+    # dwhip_params.method takes the dw_method values themselves
+    if int(s.method) not in (DWHIP_METHOD_SHB, DWHIP_METHOD_RL):
         raise ValueError("method %r has no GPU path (SHB and RL do)" % (s.method,))
+    p.method = int(s.method)
     return p
 
 
@@ -110,7 +108,10 @@ def _finish(s, st, err):
     n = st.iterations
     s.err_trace = err[:n].copy()
     s.stats = _stats_dict(st)
-    if s.bg_auto and n > 0:
+    # the background is resolved at set-up (method_shb.c:108-117), before the
+    # loop, so it is written back even when no iteration ran; with nIter 0
+    # the method returns first and st.bg is the caller's own value
+    if s.bg_auto:
         s.bg = float(st.bg)
 
 
@@ -233,7 +234,14 @@ def psf_autocrop(psf, imshape, s: DwOpts):
     """psf_autocrop (src/dw.c:1973-1997): crop to at most 2*dim-1 per axis
     (psf_autocrop_byImage, dw.c:1764-1855), then drop outer x/y planes
     whose plane sum is below xycropfactor * the largest plane sum
-    (psf_autocrop_XY, dw.c:1857-1971).  Plane sums are taken in float64."""
+    (psf_autocrop_XY, dw.c:1857-1971).
+
+    The crop index is integer output of float arithmetic, so it follows the
+    reference's: each x plane's sum accumulates in float, y outer and z
+    inner (dw.c:1873-1882), the threshold is the float product
+    xycropfactor * maxsum, and the first plane reaching it ends the scan.
+    Where no plane reaches it (xycropfactor > 1), deconwolf reads past the
+    PSF; here that is a ValueError."""
     P, N, M = imshape
     psf = np.asarray(psf, dtype=np.float32)
     p, n, m = psf.shape
@@ -250,18 +258,35 @@ def psf_autocrop(psf, imshape, s: DwOpts):
             p0, p1 = (p - popt) // 2, p1 - (p - popt) // 2
         psf = np.ascontiguousarray(psf[p0:p1 + 1, n0:n1 + 1, m0:m1 + 1])
     if s.borderQuality > 0 and s.xycropfactor > 0:
-        sums = psf.astype(np.float64).sum(axis=(0, 1))  # one sum per x
-        maxsum = max(0.0, float(sums.max()))
+        sums = xplane_sums_f32(psf)
+        maxsum = np.float32(0)
+        for v in sums:  # sum > maxsum ? maxsum = sum : 0 (dw.c:1883)
+            if v > maxsum:
+                maxsum = v
+        thr = np.float32(s.xycropfactor) * maxsum
         first = -1
-        ssum = 0.0
-        while ssum < s.xycropfactor * maxsum:
+        ssum = np.float32(0)
+        while ssum < thr:
             first += 1
-            ssum = float(sums[first])
+            if first >= sums.size:
+                raise ValueError("psf_autocrop: no x plane reaches xycropfactor * the largest plane sum")
+            ssum = sums[first]
         if first >= 1:
             pM = psf.shape[2]
             pN = psf.shape[1]
             psf = np.ascontiguousarray(psf[:, first:pN - first, first:pM - first])
     return psf
+
+
+def xplane_sums_f32(psf):
+    """Sum of each x plane of a (pP, pN, pM) PSF accumulated in float32 in
+    deconwolf's order, y outer and z inner (psf_autocrop_XY, dw.c:1873-1882):
+    one sequential float32 accumulation per plane."""
+    a = np.ascontiguousarray(np.asarray(psf, dtype=np.float32).transpose(2, 1, 0))  # [x][y][z]
+    m = a.shape[0]
+    if a.size == 0:
+        return np.zeros(m, np.float32)
+    return np.cumsum(a.reshape(m, -1), axis=1, dtype=np.float32)[:, -1].copy()
 
 
 def deconvolve(image, psf, iterations, bq=2, verbose=0, opts: Optional[DwOpts] = None):

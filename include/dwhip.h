@@ -40,12 +40,13 @@ extern "C" {
 /* dw_metric (src/dw.h:74-77) */
 #define DWHIP_METRIC_MSE 0
 #define DWHIP_METRIC_IDIV 1
+/* deconvolution method, the values of dw_method (src/dw.h:52-60:
+ * DW_METHOD_RL = 0, DW_METHOD_ID = 1, DW_METHOD_SHB = 2): plain
+ * Richardson-Lucy of method_rl.c or the scaled heavy-ball loop of
+ * method_shb.c.  DW_METHOD_ID has no GPU path (DWHIP_ERR_ARG). */
+#define DWHIP_METHOD_RL 0
+#define DWHIP_METHOD_SHB 2
 /* dw_start_condition (src/dw.h:64-72) */
-/* deconvolution method (dw_method, src/dw.h:52-60): the scaled heavy-ball
- * loop of method_shb.c or plain Richardson-Lucy of method_rl.c */
-#define DWHIP_METHOD_SHB 0
-#define DWHIP_METHOD_RL 1
-
 #define DWHIP_START_FLAT 0
 #define DWHIP_START_LP 1
 #define DWHIP_START_IDENTITY 2
