@@ -32,21 +32,44 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# served stream: cap on the transform voxels (millions) iterating together
+# served stream: cap on the transform voxels (millions) iterating together,
+# and the aging of least-work-first admission
 SERVE_MVOX = 320
+SERVE_AGING_MS = 1000
 # job groups of the engine runs (--engine-groups): two groups on two streams
 # measured 1-2 % faster on the c3 tiles and the FIFO stream
 ENGINE_GROUPS = 2
-# iterations of the job the CPU baseline times (about 10 s on 16 host threads)
-CPU_SAMPLE_ITERS = 16
+# iterations of the job the CPU baseline times: the whole c2 job (about 35 s
+# on 16 host threads); fewer would be extrapolated to the whole job
+CPU_SAMPLE_ITERS = 50
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "shb_c2_pmc_traffic.md")
 METRIC = "SHB voxel-iter/s batched-served vs plain per-image; p50/p99 job latency"
 
 
+def host_cores():
+    """(cores this process may use, os.cpu_count()): the CPU affinity mask,
+    further limited by a cgroup CPU quota when one is set."""
+    total = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = total
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, -(-int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, n), total
+
+
 def cpu_threads():
-    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    return max(1, min(n, 16))
+    """All the host cores this process may use (OMP_NUM_THREADS, when set,
+    says how many the host's operator gives a job)."""
+    n, _ = host_cores()
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, env) if env > 0 else n)
 
 
 def run_cpu_baseline(npz, iters_full, threads, timeout_s):
@@ -67,6 +90,9 @@ def run_cpu_baseline(npz, iters_full, threads, timeout_s):
     line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
     d = json.loads(line)
     d["wall_s"] = time.time() - t
+    usable, nproc = host_cores()
+    d["host_cpus"] = nproc
+    d["host_cpus_usable"] = usable
     return d
 
 
@@ -83,15 +109,35 @@ def profiled_traffic(kernel):
     return None
 
 
+def preplan(shapes):
+    """Measures the transform plans of every distinct work size among
+    (image shape, PSF shape) pairs before a timed region (dwhip_plan_tune,
+    FFTW_MEASURE's analogue; cached, so the timed jobs find them); returns
+    the seconds it took."""
+    import deconwolf_amd as dw
+    t = time.time()
+    done = set()
+    for ims, ps in shapes:
+        w, _, rc = dw.work_size(ims, ps)
+        if rc == 0 and w not in done:
+            done.add(w)
+            try:
+                dw.plan_tune(w)
+            except dw.DwhipError:
+                pass  # sizes the fused passes cannot run keep the any-size path's one plan
+    return time.time() - t
+
+
 def _pct(xs, p):
     xs = sorted(xs)
     return xs[min(len(xs) - 1, int(round(p / 100.0 * (len(xs) - 1))))]
 
 
-def run_stream(n, rate):
+def run_stream(n, rate, xy=None, fifo_leg=True):
     """The c4 stream served by the batched engine and, for comparison, by
     the plain per-image path (FIFO, one job at a time), both against the
-    same arrival times; inputs are generated and resident first."""
+    same arrival times; inputs are generated and resident first.  xy: the
+    XY sizes to draw from (default the configuration's five)."""
     import torch
     import deconwolf_amd as dw
     from deconwolf_amd import workloads
@@ -99,9 +145,10 @@ def run_stream(n, rate):
     from deconwolf_amd.serve import Engine, serve
 
     t = time.time()
-    jobs = workloads.make_stream(n, rate)
+    jobs = workloads.make_stream(n, rate, xy=xy)
     torch.cuda.synchronize()
     t_gen = time.time() - t
+    t_plan = preplan([(tuple(j["im"].shape), tuple(j["psf"].shape)) for j in jobs])
     work = sum(j["shape"][0] * j["shape"][1] * j["shape"][2] * j["iters"] for j in jobs)
     arrivals = [j["arrival"] for j in jobs]
 
@@ -126,7 +173,8 @@ def run_stream(n, rate):
     budget = max(4.0, 0.8 * free / (1 << 30))
 
     def run_engine(order, mvox):
-        eng = Engine(hbm_budget_gb=budget, max_active=64, order=order, active_mvox=mvox, groups=ENGINE_GROUPS)
+        eng = Engine(hbm_budget_gb=budget, max_active=64, order=order, active_mvox=mvox, groups=ENGINE_GROUPS,
+                     aging_ms=SERVE_AGING_MS if order == _lib.ORDER_SHORTEST else 0)
         reqs = [(i, j["im"], j["psf"], opts(j)) for i, j in enumerate(jobs)]
         times = serve(eng, reqs, arrivals)
         eng.close()
@@ -136,20 +184,24 @@ def run_stream(n, rate):
     # served: least work first, at most SERVE_MVOX transform voxels iterating
     # together (a c2-sized job is 213 M); also FIFO with everything admitted
     lat_s, served_s = run_engine(_lib.ORDER_SHORTEST, SERVE_MVOX)
-    lat_f, fifo_s = run_engine(_lib.ORDER_FIFO, 0)
+    lat_f, fifo_s = run_engine(_lib.ORDER_FIFO, 0) if fifo_leg else (lat_s, served_s)
+    del jobs
+    torch.cuda.empty_cache()
     return {
-        "workload": workloads.describe_stream(n, rate),
+        "workload": workloads.describe_stream(n, rate, xy),
         "voxel_iters": work,
         "served_value": work / served_s,
         "plain_value": work / plain_s,
         "served_vs_plain": plain_s / served_s,
         "served_p50_ms": 1e3 * _pct(lat_s, 50), "served_p99_ms": 1e3 * _pct(lat_s, 99),
         "plain_p50_ms": 1e3 * _pct(lat_p, 50), "plain_p99_ms": 1e3 * _pct(lat_p, 99),
-        "served_policy": "least work first, at most %d M transform voxels iterating together" % SERVE_MVOX,
+        "served_policy": "least work first (work / (1 + waited / %d ms)), at most %d M transform voxels iterating "
+                         "together" % (SERVE_AGING_MS, SERVE_MVOX),
         "served_fifo_all_value": work / fifo_s,
         "served_fifo_all_p50_ms": 1e3 * _pct(lat_f, 50), "served_fifo_all_p99_ms": 1e3 * _pct(lat_f, 99),
         "engine_hbm_budget_gb": budget,
         "input_gen_s": t_gen,
+        "planning_s": t_plan,
     }
 
 
@@ -169,7 +221,12 @@ def run_tiled():
     c = workloads.C3
     P, N, M = im.shape
     work = M * N * P * s.nIter
-    res = {"workload": workloads.describe_c3(), "input_gen_s": t_gen}
+    shapes = []
+    for tl in tiling.plan(M, N, P, c["tile"], c["overlap"]):
+        ts = (tl.xsize[2], tl.xsize[1], tl.xsize[0])
+        shapes.append((ts, tiling.tile_psf(psf, ts, s).shape))
+    t_plan = preplan(shapes)
+    res = {"workload": workloads.describe_c3(), "input_gen_s": t_gen, "planning_s": t_plan}
     for name, eng in (("plain_tiles", None), ("engine_tiles", "engine")):
         e = Engine(hbm_budget_gb=64.0, max_active=64, groups=ENGINE_GROUPS) if eng else None
         torch.cuda.synchronize()
@@ -202,6 +259,7 @@ def run_c5(fields=1):
         jobs += workloads.make_c5(f)
     torch.cuda.synchronize()
     res = {"workload": workloads.describe_c5(fields), "input_gen_s": time.time() - t, "jobs": len(jobs)}
+    res["planning_s"] = preplan([(tuple(j["im"].shape), tuple(j["psf"].shape)) for j in jobs])
     work = sum(j["im"].numel() * j["opts"].nIter for j in jobs)
     outs = [torch.empty_like(j["im"]) for j in jobs]
     torch.cuda.synchronize()
@@ -248,6 +306,12 @@ def main():
     ap.add_argument("--c5-fields", type=int, default=1, help="fields of the c5 sample (4 channels each); 0 skips it")
     ap.add_argument("--engine-groups", type=int, default=ENGINE_GROUPS, choices=(1, 2),
                     help="job groups of the engine runs (2: two streams whose passes overlap)")
+    ap.add_argument("--sweep", default="", metavar="RATES",
+                    help="only run the c4 stream at these Poisson rates (jobs/s, comma-separated), "
+                         "served and plain, with min(500, rate * --sweep-seconds) jobs each, and print one JSON "
+                         "line with the table")
+    ap.add_argument("--sweep-seconds", type=float, default=20.0)
+    ap.add_argument("--sweep-xy", default="", help="XY sizes of the sweep's jobs (comma-separated; default all five)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy a GIB-sized float32 buffer on the GPU 3 times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -262,6 +326,20 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 3}))
+        return
+    if a.sweep:
+        if a.gpus != 1:
+            sys.exit(2)
+        xy = tuple(int(v) for v in a.sweep_xy.split(",")) if a.sweep_xy else None
+        rows = []
+        for r in (float(v) for v in a.sweep.split(",")):
+            n = int(min(500, max(8, round(r * a.sweep_seconds))))
+            t0 = time.time()
+            row = run_stream(n, r, xy=xy, fifo_leg=False)
+            row.update(rate=r, jobs=n, wall_s=time.time() - t0)
+            print(json.dumps({"sweep_row": row}), flush=True)
+            rows.append(row)
+        print(json.dumps({"metric": METRIC, "sweep": rows, "engine_groups": ENGINE_GROUPS}))
         return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs would shard one per GPU "
@@ -299,6 +377,7 @@ def main():
         dw.deconvolve_shb_hip_device(dim, dpsf, st, out=out)
         return st
 
+    t_plan = preplan([(im.shape, psf.shape)])
     for _ in range(a.warmup):
         st = step()
     torch.cuda.synchronize()
@@ -383,6 +462,7 @@ def main():
         "engine_groups": ENGINE_GROUPS,
         "kernels": kernels,
         "input_gen_s": t_gen,
+        "planning_s": t_plan,
     }
     print(json.dumps(res))
 

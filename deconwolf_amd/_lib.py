@@ -26,6 +26,15 @@ ORDER_FIFO, ORDER_SHORTEST = 0, 1
 DWHIP_METHOD_RL, DWHIP_METHOD_SHB = 0, 2
 # dw_method (src/dw.h:52-60), the values of DwOpts.method
 METHOD_RL, METHOD_ID, METHOD_SHB = 0, 1, 2
+# dwhip_params.planning
+PLAN_RULE, PLAN_MEASURE = 0, 1
+# FFTW planner flags, the values of DwOpts.fftw3_planning (src/dw.c:400,1068)
+FFTW_MEASURE, FFTW_ESTIMATE = 0, 1 << 6
+
+
+# dwhip_params.hook: void (*)(void *user, int iter, float error, const float *x, int64 nx, ny, nz)
+HOOK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                        ctypes.c_int64, ctypes.c_int64, ctypes.c_int64)
 
 
 class DwhipParams(ctypes.Structure):
@@ -46,6 +55,11 @@ class DwhipParams(ctypes.Structure):
         ("work", ctypes.c_int64 * 3),
         ("verbosity", ctypes.c_int),
         ("method", ctypes.c_int),
+        ("planning", ctypes.c_int),
+        ("hook", HOOK),
+        ("hook_user", ctypes.c_void_p),
+        ("hook_every", ctypes.c_int),
+        ("hook_full", ctypes.c_int),
     ]
 
 
@@ -69,7 +83,9 @@ class EngineConfig(ctypes.Structure):
         ("order", ctypes.c_int),
         ("active_mvox", ctypes.c_int),
         ("groups", ctypes.c_int),
-        ("reserved", ctypes.c_int * 3),
+        ("rounds_per_step", ctypes.c_int),
+        ("aging_ms", ctypes.c_int),
+        ("reserved", ctypes.c_int * 1),
     ]
 
 
@@ -97,6 +113,9 @@ SIGNATURES = [
     ("dwhip_init", _int, []),
     ("dwhip_work_size", _int, [_i64, _i64, _i64, _i64, _i64, _i64, _int, _P64, _P64]),
     ("dwhip_job_bytes", ctypes.c_size_t, [_P64, _i64, _i64, _i64, _i64, _i64, _i64, _int, _int]),
+    ("dwhip_plan_tune", _int, [_P64, _int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    ("dwhip_plan_cache_path", _int, [ctypes.c_char_p]),
+    ("dwhip_plan_get", _int, [_P64, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     ("dwhip_shb_run", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                              ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats)]),
     ("dwhip_shb_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,

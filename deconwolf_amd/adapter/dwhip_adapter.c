@@ -19,6 +19,11 @@ void dwhip_adapter_params(const dwhip_dwopts_view *v, dwhip_params *p)
     p->start_image = NULL;
     p->work[0] = p->work[1] = p->work[2] = 0;
     p->verbosity = v->verbosity > 2 ? 1 : 0;
+    p->planning = (unsigned)v->fftw3_planning == DWHIP_FFTW_ESTIMATE ? DWHIP_PLAN_RULE : DWHIP_PLAN_MEASURE;
+    p->hook = NULL;
+    p->hook_user = NULL;
+    p->hook_every = 0;
+    p->hook_full = 0;
 }
 
 int dwhip_adapter_trace_len(const dwhip_dwopts_view *v)

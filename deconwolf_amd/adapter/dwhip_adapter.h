@@ -31,7 +31,12 @@ typedef struct {
     int metric;          /* dw_metric */
     int start_condition; /* dw_start_condition */
     int verbosity;
+    int fftw3_planning;  /* FFTW planner flag: FFTW_ESTIMATE plans by rule, others measure */
 } dwhip_dwopts_view;
+
+/* FFTW_ESTIMATE of fftw3.h, the flag dw_opts.fftw3_planning holds for
+ * --noplan (src/dw.c:1068) */
+#define DWHIP_FFTW_ESTIMATE (1U << 6)
 
 void dwhip_adapter_params(const dwhip_dwopts_view *v, dwhip_params *p);
 

@@ -24,9 +24,59 @@
 #include <string.h>
 
 #include "method_shb_hip.h"
+#include "dw_util.h"
 #include "fim.h"
+#include "fim_tiff.h"
 #include "dwhip.h"
 #include "dwhip_adapter.h"
+
+/* The side outputs deconvolve_shb writes from inside its loop, fed by
+ * libdwhip's per-iteration hook:
+ *   --iterdump k: the guess at the top of every k-th iteration, cropped,
+ *     offset removed, as itd%05d.tif (method_shb.c:302-322; the reference
+ *     then projects im instead of the dumped copy, :309, a no-op here);
+ *   --tsv: iteration, seconds since the start, the iteration's error and
+ *     KL = 0 (benchmark_write, dw.c:1339-1370; s->ref, the only source of a
+ *     KL term, is set by no deconwolf caller);
+ *   --fulldump: fulldump.tif, the returned guess, and fullPSF.tif, the
+ *     shifted PSF, both at the transform size (method_shb.c:205-209,
+ *     :407-411; with border quality 2 that size is the padded one, so the
+ *     band outside the image differs from deconwolf's dump). */
+typedef struct {
+    dw_opts *s;
+    ftif_t *ftif;
+} dump_ctx;
+
+static void on_iteration(void *user, int iter, float error, const float *x, int64_t nx, int64_t ny, int64_t nz)
+{
+    dump_ctx *c = (dump_ctx *)user;
+    dw_opts *s = c->s;
+    if (iter >= 0) {
+        if (x != NULL && s->iterdump > 0) {
+            float *temp = fim_copy(x, nx * ny * nz);
+            if (s->offset > 0)
+                fim_add_scalar(temp, nx * ny * nz, -s->offset);
+            char *outname = gen_iterdump_name(s, iter);
+            if (s->outFormat == 32)
+                fim_tiff_write_float(c->ftif, outname, temp, NULL, nx, ny, nz);
+            else
+                fim_tiff_write(c->ftif, outname, temp, NULL, nx, ny, nz);
+            free(outname);
+            fim_free(temp);
+        }
+        if (s->tsv != NULL) {
+            struct timespec now;
+            dw_gettime(&now);
+            fprintf(s->tsv, "%d\t%f\t%f\t%f\n", iter, clockdiff(&now, &s->tstart), (double)error, 0.0);
+        }
+    } else if (iter == -1) {
+        printf("Dumping to fulldump.tif\n");
+        fim_tiff_write(c->ftif, "fulldump.tif", x, NULL, nx, ny, nz);
+    } else if (iter == -2) {
+        printf("Dumping to fullPSF.tif\n");
+        fim_tiff_write_float(c->ftif, "fullPSF.tif", x, NULL, nx, ny, nz);
+    }
+}
 
 static float *run_hip(int rl, float *restrict im, const int64_t M, const int64_t N, const int64_t P,
                       float *restrict psf, const int64_t pM, const int64_t pN, const int64_t pP,
@@ -52,6 +102,7 @@ static float *run_hip(int rl, float *restrict im, const int64_t M, const int64_t
     v.metric = (int)s->metric;
     v.start_condition = (int)s->start_condition;
     v.verbosity = s->verbosity;
+    v.fftw3_planning = s->fftw3_planning;
 
     dwhip_params p;
     dwhip_adapter_params(&v, &p);
@@ -75,6 +126,17 @@ static float *run_hip(int rl, float *restrict im, const int64_t M, const int64_t
         p.start_image = im_lp;
     }
 
+    dump_ctx dc;
+    dc.s = s;
+    dc.ftif = NULL;
+    if (s->iterdump > 0 || s->tsv != NULL || s->fulldump) {
+        dc.ftif = fim_tiff_new(s->log, s->verbosity);
+        p.hook = on_iteration;
+        p.hook_user = &dc;
+        p.hook_every = s->iterdump;
+        p.hook_full = s->fulldump ? 1 : 0;
+    }
+
     const int ntrace = dwhip_adapter_trace_len(&v);
     float *trace = calloc((size_t)ntrace, sizeof(float));
     float *out = fim_malloc(M * N * P * sizeof(float));
@@ -93,6 +155,8 @@ static float *run_hip(int rl, float *restrict im, const int64_t M, const int64_t
     fim_free(psf);
     if (im_lp != NULL)
         fim_free(im_lp);
+    if (dc.ftif != NULL)
+        fim_tiff_destroy(dc.ftif);
     if (rc != DWHIP_OK) {
         fprintf(stderr, "%s: %s\n", tag, dwhip_strerror(rc));
         if (s->log != NULL)

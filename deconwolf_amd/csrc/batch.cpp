@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <map>
 #include <vector>
@@ -86,6 +87,7 @@ struct Arena {
 
 struct EngJob {
     DwhipJob j;
+    long long serial = 0;    // unique per engine: identifies the job in graph_jobs
     dwhip_params prm;
     int64_t tag = 0;
     float* out = nullptr;
@@ -95,7 +97,13 @@ struct EngJob {
     float last_err = 1, prev_err = 1;
     std::vector<float> err;  // error trace, filled at completion
     float bg = 0;
+    double submitted = 0;    // host clock (s) at submission, for aging
 };
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 // A device table: BatchDev at the start, the JobDev array 64 bytes in.
 struct Table {
@@ -125,7 +133,17 @@ struct dwhip_engine {
     std::vector<EngJob*> active;
     std::map<int64_t, EngJob*> done;
     Table tsetup, tweights, tae, tfh, tae2, tfh2;
+    // graph of one full round of the current active set (both groups),
+    // replayed for the rounds of a step before its last; its tables stay
+    // as captured until the active set changes
+    Table gae, gae2;
+    hipGraphExec_t graph = nullptr;
+    std::vector<long long> graph_jobs[2];  // serials of the jobs the graph was captured for
+    long long next_serial = 1;
+    hipStream_t cs2 = nullptr;     // second branch of a two-group capture
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     long long rounds = 0;
+    long long graph_launches = 0;
 };
 
 namespace {
@@ -221,18 +239,26 @@ double job_work(const EngJob* j)
 // Admission: while the active set has room (max_active, the voxel cap and
 // the arena), the next job in the configured order that fits.  FIFO stops
 // at the first job that does not fit, so later jobs never overtake it;
-// shortest-first takes the least work among those that fit.
+// shortest-first takes the least work among those that fit, the work of a
+// job that has waited t seconds counted as work / (1 + t / aging), so that
+// a large job is not passed over for ever (aging_ms 0: no aging).
 int admit(dwhip_engine* E, std::vector<EngJob*>& fresh)
 {
     const double cap = (double)E->cfg.active_mvox * 1e6;
     double vox = 0;
     for (EngJob* j : E->active) vox += job_voxels(j);
+    const double t = now_s();
+    const double aging = E->cfg.aging_ms > 0 ? 1e-3 * E->cfg.aging_ms : 0;
+    auto score = [&](const EngJob* j) {
+        const double w = job_work(j);
+        return aging > 0 ? w / (1.0 + (t - j->submitted) / aging) : w;
+    };
     while (!E->queue.empty() && (int)E->active.size() < E->cfg.max_active) {
         std::vector<size_t> order(E->queue.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = i;
         if (E->cfg.order == DWHIP_ORDER_SHORTEST)
             std::stable_sort(order.begin(), order.end(),
-                             [&](size_t a, size_t b) { return job_work(E->queue[a]) < job_work(E->queue[b]); });
+                             [&](size_t a, size_t b) { return score(E->queue[a]) < score(E->queue[b]); });
         bool taken = false;
         for (size_t i : order) {
             EngJob* j = E->queue[i];
@@ -260,8 +286,17 @@ int setup(dwhip_engine* E, const std::vector<EngJob*>& fresh)
 {
     if (fresh.empty()) return DWHIP_OK;
     hipStream_t s = E->s;
-    RCHK(table_build(&E->tsetup, fresh, s));
-    for (int i = 0; i < (int)fresh.size(); ++i) {
+    // jobs on the any-size path set up one by one; the others batched
+    std::vector<EngJob*> fast;
+    for (EngJob* j : fresh) {
+        if (j->j.generic)
+            RCHK(dwhip_job_setup(&j->j, s));
+        else
+            fast.push_back(j);
+    }
+    if (fast.empty()) return DWHIP_OK;
+    RCHK(table_build(&E->tsetup, fast, s));
+    for (int i = 0; i < (int)fast.size(); ++i) {
         ECHK(dwhip_launch_stats(E->tsetup.jd(), i, DWHIP_STAT_BLOCKS, s));
         ECHK(dwhip_launch_start(E->tsetup.jd(), i, 1024, s));
     }
@@ -269,7 +304,7 @@ int setup(dwhip_engine* E, const std::vector<EngJob*>& fresh)
     RCHK(table_pass(E->tsetup, false, CM_YFWD, s));
     RCHK(table_pass(E->tsetup, false, CM_ZOTF, s));
     std::vector<EngJob*> wj;
-    for (EngJob* j : fresh)
+    for (EngJob* j : fast)
         if (j->j.d.has_w) wj.push_back(j);
     if (!wj.empty()) {
         RCHK(table_build(&E->tweights, wj, s));
@@ -292,6 +327,7 @@ void dwhip_engine_config_default(dwhip_engine_config* c)
     c->hbm_budget = (size_t)32 << 30;
     c->max_active = 64;
     c->groups = 1;
+    c->rounds_per_step = 8;
 }
 
 int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engine** out)
@@ -301,6 +337,7 @@ int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engi
     if ((cfg->order != DWHIP_ORDER_FIFO && cfg->order != DWHIP_ORDER_SHORTEST) || cfg->active_mvox < 0)
         return DWHIP_ERR_ARG;
     if (cfg->groups != 0 && cfg->groups != 1 && cfg->groups != 2) return DWHIP_ERR_ARG;  // 0 reads as 1
+    if (cfg->rounds_per_step < 0 || cfg->aging_ms < 0) return DWHIP_ERR_ARG;             // 0 reads as 1
     int rc = dwhip_init();
     if (rc != DWHIP_OK) return rc;
     dwhip_engine* E = new dwhip_engine();
@@ -323,14 +360,23 @@ int dwhip_engine_create(const dwhip_engine_config* cfg, void* stream, dwhip_engi
     E->arena.init(cfg->hbm_budget);
     const int cap = cfg->max_active;
     if (table_alloc(&E->tsetup, cap) || table_alloc(&E->tweights, cap) || table_alloc(&E->tae, cap) ||
-        table_alloc(&E->tfh, cap)) {
+        table_alloc(&E->tfh, cap) || table_alloc(&E->gae, cap)) {
         dwhip_engine_destroy(E);
         return DWHIP_ERR_NOMEM;
     }
+    if (hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming) != hipSuccess) {
+        dwhip_engine_destroy(E);
+        return DWHIP_ERR_HIP;
+    }
     if (cfg->groups == 2) {
-        if (table_alloc(&E->tae2, cap) || table_alloc(&E->tfh2, cap)) {
+        if (table_alloc(&E->tae2, cap) || table_alloc(&E->tfh2, cap) || table_alloc(&E->gae2, cap)) {
             dwhip_engine_destroy(E);
             return DWHIP_ERR_NOMEM;
+        }
+        if (hipStreamCreateWithFlags(&E->cs2, hipStreamNonBlocking) != hipSuccess) {
+            dwhip_engine_destroy(E);
+            return DWHIP_ERR_HIP;
         }
         // default priority, like every stream here
         if (hipStreamCreateWithFlags(&E->s2, hipStreamNonBlocking) != hipSuccess ||
@@ -357,6 +403,12 @@ void dwhip_engine_destroy(dwhip_engine* E)
     table_free(&E->tfh);
     table_free(&E->tae2);
     table_free(&E->tfh2);
+    table_free(&E->gae);
+    table_free(&E->gae2);
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    if (E->cs2) (void)hipStreamDestroy(E->cs2);
+    if (E->ev_fork) (void)hipEventDestroy(E->ev_fork);
+    if (E->ev_join) (void)hipEventDestroy(E->ev_join);
     if (E->ev_setup) (void)hipEventDestroy(E->ev_setup);
     if (E->s2) (void)hipStreamDestroy(E->s2);
     if (E->arena_mem) (void)hipFree(E->arena_mem);
@@ -377,7 +429,19 @@ int dwhip_engine_submit(dwhip_engine* E, const float* im, int64_t M, int64_t N, 
     const bool fixed = prm->iter_type == DWHIP_ITER_FIXED;
     if (prm->niter < 1 || (!fixed && prm->maxiter < 1)) return DWHIP_ERR_ARG;
     if (prm->method != DWHIP_METHOD_SHB && prm->method != DWHIP_METHOD_RL) return DWHIP_ERR_ARG;
+    if (prm->planning == DWHIP_PLAN_MEASURE) {
+        // the transform size's measured plan, on its first use
+        int64_t w[3], ref[3];
+        if (dwhip_work_size(M, N, P, pM, pN, pP, prm->border_quality, w, ref) == DWHIP_OK) {
+            for (int d = 0; d < 3; ++d)
+                if (prm->work[d] > 0) w[d] = prm->work[d];
+            const int trc = dwhip_plan_ensure(w, prm->method);
+            if (trc != DWHIP_OK) return trc;
+        }
+    }
     EngJob* j = new EngJob();
+    j->serial = E->next_serial++;
+    j->submitted = now_s();
     j->prm = *prm;
     j->tag = tag;
     j->out = out;
@@ -435,14 +499,22 @@ static void split_groups(const std::vector<EngJob*>& act, std::vector<EngJob*>* 
 
 // One iteration of every job of a group on stream s; the adjoint half
 // (F, G, H) only for jobs with a further iteration to feed.
-static int group_round(Table* tae, Table* tfh, const std::vector<EngJob*>& jobs, hipStream_t s)
+static int group_round(Table* tae, Table* tfh, const std::vector<EngJob*>& all, hipStream_t s)
 {
-    if (jobs.empty()) return DWHIP_OK;
-    std::vector<EngJob*> fh;
-    for (EngJob* j : jobs) {
+    if (all.empty()) return DWHIP_OK;
+    std::vector<EngJob*> jobs, fh;
+    for (EngJob* j : all) {
         const bool fixed = j->prm.iter_type == DWHIP_ITER_FIXED;
-        if (!fixed || j->j.d.rl || j->j.host_iter + 1 < j->prm.niter) fh.push_back(j);
+        const bool full = !fixed || j->j.d.rl || j->j.host_iter + 1 < j->prm.niter;
+        if (j->j.generic) {
+            // the any-size path runs its own passes, one job at a time
+            RCHK(dwhip_generic_round(&j->j, full, s));
+            continue;
+        }
+        jobs.push_back(j);
+        if (full) fh.push_back(j);
     }
+    if (jobs.empty()) return DWHIP_OK;
     RCHK(table_build(tae, jobs, s));
     RCHK(table_build(tfh, fh, s));
     RCHK(table_pass(*tae, true, XM_UPDATE, s));
@@ -454,6 +526,85 @@ static int group_round(Table* tae, Table* tfh, const std::vector<EngJob*>& jobs,
     RCHK(table_pass(*tfh, false, CM_ZCONVJ, s));
     RCHK(table_pass(*tfh, false, CM_YINV, s));
     ECHK(dwhip_launch_round_end(tae->bd(), tae->jd(), tae->n, 1, s));
+    return DWHIP_OK;
+}
+
+// The passes of one full round (every job transforms its adjoint half) over
+// table t, which serves as both the A-E and the F-H table.
+static int full_round(const Table& t, hipStream_t s)
+{
+    RCHK(table_pass(t, true, XM_UPDATE, s));
+    RCHK(table_pass(t, false, CM_YFWD, s));
+    RCHK(table_pass(t, false, CM_ZCONV, s));
+    RCHK(table_pass(t, false, CM_YINV_IMG, s));
+    RCHK(table_pass(t, true, XM_RATIO, s));
+    RCHK(table_pass(t, false, CM_YFWD_IMG, s));
+    RCHK(table_pass(t, false, CM_ZCONVJ, s));
+    RCHK(table_pass(t, false, CM_YINV, s));
+    ECHK(dwhip_launch_round_end(t.bd(), t.jd(), t.n, 1, s));
+    return DWHIP_OK;
+}
+
+// Capture body: group 0 on the capture stream, group 1 (if any) on a
+// branch forked from it and joined back.
+static int capture_round(void* ctx, hipStream_t cs)
+{
+    dwhip_engine* E = (dwhip_engine*)ctx;
+    const bool two = E->gae2.n > 0;
+    if (two) {
+        ECHK(hipEventRecord(E->ev_fork, cs));
+        ECHK(hipStreamWaitEvent(E->cs2, E->ev_fork, 0));
+    }
+    RCHK(full_round(E->gae, cs));
+    if (two) {
+        RCHK(full_round(E->gae2, E->cs2));
+        ECHK(hipEventRecord(E->ev_join, E->cs2));
+        ECHK(hipStreamWaitEvent(cs, E->ev_join, 0));
+    }
+    return DWHIP_OK;
+}
+
+// Rounds this step runs: with every active job in FIXED mode, up to the
+// next completion (the fewest rounds any job has left), at most
+// rounds_per_step; one round otherwise (REL/ABS decide after each round),
+// and while kernels are profiled (events per pass).
+static int step_rounds(const dwhip_engine* E)
+{
+    const int cap = std::max(1, E->cfg.rounds_per_step);
+    if (cap == 1 || dwhip_prof_on() || dwhip_graph_mode() == 0) return 1;
+    int r = cap;
+    for (const EngJob* j : E->active) {
+        if (j->prm.iter_type != DWHIP_ITER_FIXED || j->j.generic) return 1;
+        r = std::min(r, j->prm.niter - j->j.host_iter);
+    }
+    return std::max(1, r);
+}
+
+// Replays `n` full rounds of the active groups on E->s from one captured
+// round, capturing it again when the groups changed.
+static int graph_rounds(dwhip_engine* E, const std::vector<EngJob*>* grp, int n)
+{
+    if (n <= 0) return DWHIP_OK;
+    std::vector<long long> ids[2];
+    for (int g = 0; g < 2; ++g)
+        for (const EngJob* j : grp[g]) ids[g].push_back(j->serial);
+    if (!E->graph || ids[0] != E->graph_jobs[0] || ids[1] != E->graph_jobs[1]) {
+        if (E->graph) {
+            ECHK(hipStreamSynchronize(E->s));  // a previous step's replays may still read gae
+            (void)hipGraphExecDestroy(E->graph);
+            E->graph = nullptr;
+        }
+        RCHK(table_build(&E->gae, grp[0], E->s));
+        if (E->gae2.cap) RCHK(table_build(&E->gae2, grp[1], E->s));
+        RCHK(dwhip_capture(capture_round, E, &E->graph));
+        E->graph_jobs[0] = ids[0];
+        E->graph_jobs[1] = ids[1];
+    }
+    for (int k = 0; k < n; ++k) ECHK(hipGraphLaunch(E->graph, E->s));
+    E->graph_launches += n;
+    for (int g = 0; g < 2; ++g)
+        for (EngJob* j : grp[g]) j->j.host_iter += n;
+    E->rounds += n;
     return DWHIP_OK;
 }
 
@@ -484,16 +635,19 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         ECHK(hipStreamSynchronize(s));
         return finish_jobs(E, finished, done_tags, max_done, n_done);
     }
-    // the jobs of this round, in one group on s or two on s and s2
+    // the jobs of this step, in one group on s or two on s and s2
     std::vector<EngJob*> grp[2];
     const bool two = E->s2 && E->active.size() >= 2;
-    if (two) {
+    if (two)
         split_groups(E->active, grp);
-        // the second group's jobs may have been set up on s in this step
+    else
+        grp[0] = E->active;
+    // all rounds but the step's last from the round graph, on s
+    RCHK(graph_rounds(E, grp, step_rounds(E) - 1));
+    if (two) {
+        // the second group's jobs may have been set up (or replayed) on s
         ECHK(hipEventRecord(E->ev_setup, s));
         ECHK(hipStreamWaitEvent(E->s2, E->ev_setup, 0));
-    } else {
-        grp[0] = E->active;
     }
     hipStream_t gs[2] = {s, two ? E->s2 : s};
     RCHK(group_round(&E->tae, &E->tfh, grp[0], gs[0]));

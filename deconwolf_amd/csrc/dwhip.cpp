@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "dwhip.h"
@@ -22,7 +23,7 @@ const char* dwhip_strerror(int code)
     switch (code) {
     case DWHIP_OK: return "ok";
     case DWHIP_ERR_ARG: return "invalid argument";
-    case DWHIP_ERR_SIZE: return "transform size not supported (needs factors 2,3,5,7 and an even x extent)";
+    case DWHIP_ERR_SIZE: return "transform size beyond the kernels' limits";
     case DWHIP_ERR_HIP: return "HIP runtime error";
     case DWHIP_ERR_NOMEM: return "device memory allocation failed";
     case DWHIP_ERR_PSF: return "the PSF maximum could not be moved to the origin";
@@ -73,6 +74,7 @@ void dwhip_params_default(dwhip_params* p, int niter)
     p->metric = DWHIP_METRIC_IDIV;
     p->start = DWHIP_START_FLAT;
     p->method = DWHIP_METHOD_SHB;
+    p->planning = DWHIP_PLAN_MEASURE;  // FFTW_MEASURE, src/dw.c:400
 }
 
 // ---------------------------------------------------------------------------
@@ -122,7 +124,8 @@ static void ref_work_size(int bq, int64_t M, int64_t N, int64_t P, int64_t pM, i
     }
 }
 
-static bool size_ok(const int64_t* w)
+// whether the fused passes transform a work size w
+bool dwhip_fast_size_ok(const int64_t* w)
 {
     if (w[0] % 2 != 0 || !is_smooth(w[0] / 2) || !is_smooth(w[1]) || !is_smooth(w[2])) return false;
     // kernel limits: at most DWHIP_XMAXE*NTHR complex per x-row tile,
@@ -157,7 +160,8 @@ int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int
     }
     if (work)
         for (int d = 0; d < 3; ++d) work[d] = w[d];
-    return size_ok(w) ? DWHIP_OK : DWHIP_ERR_SIZE;
+    // the fused passes, or else the any-size path (any factors, odd x)
+    return dwhip_fast_size_ok(w) || dwhip_generic_size_ok(w) ? DWHIP_OK : DWHIP_ERR_SIZE;
 }
 
 // Radices for a 2,3,5,7-smooth n: fewest stages, then the smallest largest
@@ -227,10 +231,51 @@ int dwhip_make_plan(int n, const std::vector<int>& r, FftPlanDev* p, std::vector
     return ns == n || n == 1 ? DWHIP_OK : DWHIP_ERR_SIZE;
 }
 
-// forward (radices R1..Rk) and inverse (Rk..R1) plans of length n
-static int make_plan_pair(int n, int axis, FftPlanDev* f, FftPlanDev* i, std::vector<float2>& tw)
+// All radix lists for n along an axis, at most DWHIP_MAX_FFT_STAGES stages
+// of radices the axis instantiates: each multiset in non-increasing and in
+// non-decreasing order, multisets ranked as the rule planner ranks them
+// (fewest stages, then smallest largest radix); the rule's plan first.
+static void all_factorisations(int n, int rmax, std::vector<int>& cur, std::vector<std::vector<int>>& out)
 {
-    std::vector<int> r = dwhip_radices(n, axis);
+    if (n == 1) {
+        if (!cur.empty()) out.push_back(cur);
+        return;
+    }
+    if ((int)cur.size() >= DWHIP_MAX_FFT_STAGES) return;
+    for (int r : kRadices) {
+        if (r > rmax || (!cur.empty() && r > cur.back()) || n % r != 0) continue;
+        cur.push_back(r);
+        all_factorisations(n / r, rmax, cur, out);
+        cur.pop_back();
+    }
+}
+
+int dwhip_radix_candidates(int n, int axis, std::vector<std::vector<int>>* out)
+{
+    out->clear();
+    if (n <= 1) return DWHIP_OK;
+    std::vector<std::vector<int>> ms;
+    std::vector<int> cur;
+    all_factorisations(n, axis_rmax(axis), cur, ms);
+    if (ms.empty()) return DWHIP_ERR_SIZE;
+    std::stable_sort(ms.begin(), ms.end(), [](const std::vector<int>& a, const std::vector<int>& b) {
+        if (a.size() != b.size()) return a.size() < b.size();
+        return *std::max_element(a.begin(), a.end()) < *std::max_element(b.begin(), b.end());
+    });
+    for (const auto& m : ms) {
+        out->push_back(m);
+        std::vector<int> rev(m.rbegin(), m.rend());
+        if (rev != m) out->push_back(rev);
+    }
+    return DWHIP_OK;
+}
+
+// forward (radices R1..Rk) and inverse (Rk..R1) plans of length n, with
+// the given radices (empty: the rule planner's)
+static int make_plan_pair(int n, int axis, FftPlanDev* f, FftPlanDev* i, std::vector<float2>& tw,
+                          const std::vector<int>* given = nullptr)
+{
+    std::vector<int> r = given && !given->empty() ? *given : dwhip_radices(n, axis);
     if (n > 1 && r.empty()) return DWHIP_ERR_SIZE;
     std::vector<int> rr(r.rbegin(), r.rend());
     int rc = dwhip_make_plan(n, r, f, tw);
@@ -251,8 +296,9 @@ static int ratio_tiles(int64_t N, int64_t P, int xrows)
     return (int)(((N + xrows - 1) / xrows) * P);
 }
 
-// Tile shapes and LDS bytes for a transform size.
-void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
+// Tile shapes and LDS bytes for a transform size; pc (optional) gives
+// smaller tiles than the rule's largest ones (a power of two for y and z).
+void dwhip_tile_config(const int64_t* w, DwhipTiles* t, const DwhipPlanChoice* pc)
 {
     const int h = (int)(w[0] / 2), ny = (int)w[1], nz = (int)w[2];
     const int hc = h + 1;
@@ -262,6 +308,11 @@ void dwhip_tile_config(const int64_t* w, DwhipTiles* t)
     const int zlim = std::min(DWHIP_ZMAXE * DWHIP_ZTHR, (int)(DWHIP_MAX_LDS_BYTES / sizeof(float2)));
     t->ycols = std::max(1, std::min(64, pow2_floor(ylim / ny)));
     t->zcols = std::max(1, std::min(64, pow2_floor(zlim / nz)));
+    if (pc) {
+        if (pc->xrows > 0) t->xrows = std::min(t->xrows, pc->xrows);
+        if (pc->ycols > 0) t->ycols = std::min(t->ycols, pow2_floor(pc->ycols));
+        if (pc->zcols > 0) t->zcols = std::min(t->zcols, pow2_floor(pc->zcols));
+    }
     t->ntile[KIND_X] = (ny * nz + t->xrows - 1) / t->xrows;
     t->ntile[KIND_Y] = ((hc + t->ycols - 1) / t->ycols) * nz;
     t->ntile[KIND_Z] = ((hc + t->zcols - 1) / t->zcols) * ny;
@@ -282,6 +333,16 @@ static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 size_t dwhip_job_bytes(const int64_t* work, int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN,
                        int64_t pP, int border_quality, int niter)
 {
+    if (!dwhip_fast_size_ok(work)) {
+        // any-size path: complex Z and K, U0, U1 (and W), partials, tables
+        const size_t V = (size_t)work[0] * work[1] * work[2];
+        size_t b = 2 * V * sizeof(float2) + (border_quality > 0 ? 3 : 2) * V * sizeof(float);
+        b += (size_t)dwhip_generic_ratio_grid() * sizeof(double);
+        b += (size_t)(niter + 1) * (sizeof(float) + sizeof(double));
+        b += DWHIP_STAT_BLOCKS * 4 * sizeof(double) + 4096 + sizeof(GenDev) + sizeof(JobDev);
+        for (int d = 0; d < 3; ++d) b += 8 * (size_t)work[d] * sizeof(float2);  // twiddles, chirps, filters
+        return b;
+    }
     const size_t h = work[0] / 2, hcs = round_up(h + 1, 16);
     const size_t V = (size_t)work[0] * work[1] * work[2];
     const size_t Vc = hcs * work[1] * work[2];
@@ -305,10 +366,74 @@ size_t dwhip_job_bytes(const int64_t* work, int64_t M, int64_t N, int64_t P, int
         if (e_ != hipSuccess) return dwhip_hip_fail(e_, #call, __LINE__); \
     } while (0)
 
+// Device blocks of the per-image path are kept between calls: a repeated
+// job reuses the block instead of a hipMalloc/hipFree of its 1-5 GB each
+// time.  One cached block per device, grown to the largest job seen and
+// held until the process ends ($DWHIP_JOB_CACHE=0 turns this off); a call
+// that finds it in use (another host thread's job) allocates its own.
+namespace {
+struct CachedBlock {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool busy = false;
+};
+constexpr int kCacheDevices = 64;
+std::mutex g_cache_mu;
+CachedBlock g_cache[kCacheDevices];
+
+bool job_cache_on()
+{
+    static const bool on = [] {
+        const char* v = getenv("DWHIP_JOB_CACHE");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+}  // namespace
+
+static hipError_t job_mem_acquire(size_t bytes, void** out)
+{
+    int dev = 0;
+    if (job_cache_on() && hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kCacheDevices) {
+        std::lock_guard<std::mutex> lock(g_cache_mu);
+        CachedBlock& c = g_cache[dev];
+        if (!c.busy) {
+            if (c.bytes < bytes) {
+                if (c.p) (void)hipFree(c.p);
+                c.p = nullptr;
+                c.bytes = 0;
+                const hipError_t e = hipMalloc(&c.p, bytes);
+                if (e != hipSuccess) {
+                    c.p = nullptr;
+                    return e;
+                }
+                c.bytes = bytes;
+            }
+            c.busy = true;
+            *out = c.p;
+            return hipSuccess;
+        }
+    }
+    return hipMalloc(out, bytes);
+}
+
+static void job_mem_release(void* p)
+{
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mu);
+        for (CachedBlock& c : g_cache)
+            if (c.p == p && c.busy) {
+                c.busy = false;
+                return;
+            }
+    }
+    (void)hipFree(p);
+}
+
 void dwhip_job_free(DwhipJob* j)
 {
     if (!j) return;
-    if (j->mem) hipFree(j->mem);
+    if (j->mem) job_mem_release(j->mem);
     j->mem = nullptr;
 }
 
@@ -333,10 +458,11 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
         if (prm->border_quality < 2)
             for (int d = 0; d < 3; ++d)
                 if (w[d] != ref[d]) return DWHIP_ERR_ARG;  // padding changes bq 0/1 results
-        if (!size_ok(w)) return DWHIP_ERR_SIZE;
+        if (!dwhip_fast_size_ok(w) && !dwhip_generic_size_ok(w)) return DWHIP_ERR_SIZE;
     } else if (rc != DWHIP_OK) {
         return rc;
     }
+    j->generic = !dwhip_fast_size_ok(w);
     if (M * N * P > (int64_t)1 << 31 || w[0] * w[1] * w[2] > (int64_t)1 << 34) return DWHIP_ERR_SIZE;
     for (int d = 0; d < 3; ++d) {
         j->work[d] = w[d];
@@ -354,7 +480,12 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.M = (int)M; d.N = (int)N; d.P = (int)P;
     d.pM = (int)pM; d.pN = (int)pN; d.pP = (int)pP;
     d.refMNP = (double)ref[0] * (double)ref[1] * (double)ref[2];
-    dwhip_tile_config(w, &j->tiles);
+    // plan: the autotuner's candidate, else a measured plan from the cache,
+    // else the rule planner's
+    DwhipPlanChoice cached;
+    const DwhipPlanChoice* pc = j->force_plan;
+    if (!pc && !j->generic && dwhip_plan_cached(w, &cached)) pc = &cached;
+    dwhip_tile_config(w, &j->tiles, pc);
     {
         // pass D transforms slabs z < P; pass F too, except for RL, whose
         // ratio is 1e-6 outside the image
@@ -382,9 +513,38 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.psf = psf_dev;
     d.start_img = start_dev;
 
+    if (j->generic) {
+        // any-size path: complex volumes Z (in C) and K of nx*ny*nz values;
+        // k_round_end adds up the ratio kernel's one partial per workgroup
+        const int grc = dwhip_generic_plan(j);
+        if (grc != DWHIP_OK) return grc;
+        j->rtw_off = 0;
+        for (int k = 0; k < KIND_N; ++k) d.ntile[k] = 0;
+        d.ntile[KIND_XR] = dwhip_generic_ratio_grid();
+        const size_t V = (size_t)d.nx * d.ny * d.nz;
+        size_t off = 0;
+        auto take = [&](size_t bytes) { size_t o = off; off = round_up(off + bytes, 256); return o; };
+        DwhipLayout& L = j->lay;
+        L.C = take(V * sizeof(float2));
+        L.K = take(V * sizeof(float2));
+        L.U0 = take(V * sizeof(float));
+        L.U1 = take(V * sizeof(float));
+        L.W = d.has_w ? take(V * sizeof(float)) : 0;
+        L.part = take((size_t)d.ntile[KIND_XR] * sizeof(double));
+        L.stat = take(DWHIP_STAT_BLOCKS * 4 * sizeof(double));
+        L.err = take((size_t)niter * sizeof(float));
+        L.errd = take((size_t)niter * sizeof(double));
+        L.tw = take(j->tw.size() * sizeof(float2));
+        L.st = take(sizeof(JobState));
+        L.tab = take(64 + round_up(sizeof(JobDev), 64) + sizeof(GenDev));
+        j->bytes = off;
+        return DWHIP_OK;
+    }
+
     std::vector<float2>& tw = j->tw;
-    if (make_plan_pair(d.h, 0, &d.pxf, &d.pxi, tw) || make_plan_pair(d.ny, 1, &d.pyf, &d.pyi, tw) ||
-        make_plan_pair(d.nz, 2, &d.pzf, &d.pzi, tw))
+    if (make_plan_pair(d.h, 0, &d.pxf, &d.pxi, tw, pc ? &pc->r[0] : nullptr) ||
+        make_plan_pair(d.ny, 1, &d.pyf, &d.pyi, tw, pc ? &pc->r[1] : nullptr) ||
+        make_plan_pair(d.nz, 2, &d.pzf, &d.pzi, tw, pc ? &pc->r[2] : nullptr))
         return DWHIP_ERR_SIZE;
     j->rtw_off = tw.size();
     for (int k = 0; k < d.h; ++k) {
@@ -447,6 +607,7 @@ int dwhip_job_bind(DwhipJob* j, void* mem, hipStream_t s)
     HIPCHK(hipMemcpyAsync(base + L.tw, j->tw.data(), j->tw.size() * sizeof(float2), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(j->bd, &j->bh, sizeof(j->bh), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(j->jd, &j->dcopy, sizeof(j->dcopy), hipMemcpyHostToDevice, s));
+    if (j->generic) return dwhip_generic_bind(j, base + L.tab + 64 + round_up(sizeof(JobDev), 64), s);
     return DWHIP_OK;
 }
 
@@ -458,7 +619,7 @@ int dwhip_job_create(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int
     int rc = dwhip_job_plan(j, im_dev, M, N, P, psf_dev, pM, pN, pP, prm, start_dev);
     if (rc != DWHIP_OK) return rc;
     void* mem = nullptr;
-    hipError_t e = hipMalloc(&mem, j->bytes);
+    hipError_t e = job_mem_acquire(j->bytes, &mem);
     if (e != hipSuccess) return e == hipErrorOutOfMemory ? DWHIP_ERR_NOMEM : DWHIP_ERR_HIP;
     rc = dwhip_job_bind(j, mem, nullptr);
     if (rc == DWHIP_OK) {
@@ -539,6 +700,7 @@ int dwhip_pass_c(int mode, const DwhipJob* j, const BatchDev* bd, const JobDev* 
 // OTF, Bertero weights and start guess for a single job (method_shb.c:108-294).
 int dwhip_job_setup(const DwhipJob* j, hipStream_t s)
 {
+    if (j->generic) return dwhip_generic_setup(j, s);
     const DwhipTiles& t = j->tiles;
     const int nblk = DWHIP_STAT_BLOCKS;
     HIPCHK(dwhip_launch_stats(j->jd, 0, nblk, s));
@@ -562,6 +724,11 @@ int dwhip_job_setup(const DwhipJob* j, hipStream_t s)
 // half (F, G, H), which only the next iteration's update consumes.
 int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s)
 {
+    if (j->generic) {
+        RCHK(dwhip_generic_round(j, full, s));
+        j->host_iter += 1;
+        return DWHIP_OK;
+    }
     const DwhipTiles& t = j->tiles;
     RCHK(dwhip_pass_x(XM_UPDATE, j, j->bd, j->jd, t.ntile[KIND_X], t.lds[KIND_X], s));
     RCHK(dwhip_pass_c(CM_YFWD, j, j->bd, j->jd, t.ntile[KIND_Y], t.lds[KIND_Y], s));
@@ -583,6 +750,7 @@ int dwhip_job_round(DwhipJob* j, bool full, hipStream_t s)
 // Its forward transform is written to C and not used.
 int dwhip_job_final_update(DwhipJob* j, const BatchDev* bd, const JobDev* jd, hipStream_t s)
 {
+    if (j->generic) return dwhip_generic_final_update(j, s);
     return dwhip_pass_x(XM_UPDATE, j, bd, jd, j->tiles.ntile[KIND_X], j->tiles.lds[KIND_X], s);
 }
 
@@ -600,34 +768,174 @@ static bool iterator_continue(const dwhip_params* p, int iter, float error, floa
     return iter < niter;
 }
 
+// One job on stream s.  mem: a device block of cap bytes for the job's
+// state, or nullptr to allocate one here for the call.  With a given block
+// nothing is allocated on the device (no timing events either: setup_ms
+// and iter_ms read 0), and a job needing more than cap bytes is refused
+// with DWHIP_ERR_NOMEM before anything is queued.
+// FIXED iterations of a job on stream s: niter rounds, the last one of SHB
+// without its adjoint half (SHB returns x_{n-1}, so that half feeds
+// nothing).  Graph mode 1 replays one captured full round for every round
+// but SHB's last; mode 2 captures the whole loop; mode 0, a job bound to a
+// caller's block (nothing may be allocated, and instantiating a graph
+// allocates) and profiled runs (events per pass) launch every pass.
+struct LoopCtx {
+    DwhipJob* job;
+    int niter;
+    int full_only;  // capture one full round
+};
+
+static int loop_body(void* ctx, hipStream_t cs)
+{
+    LoopCtx* c = (LoopCtx*)ctx;
+    if (c->full_only) return dwhip_job_round(c->job, true, cs);
+    for (int k = 0; k < c->niter; ++k) RCHK(dwhip_job_round(c->job, c->job->d.rl || k + 1 < c->niter, cs));
+    return DWHIP_OK;
+}
+
+// *exec receives the graph launched (destroyed by the caller once s has
+// run it), or nullptr.
+static int fixed_loop(DwhipJob* job, int niter, bool may_graph, hipStream_t s, hipGraphExec_t* exec_out)
+{
+    *exec_out = nullptr;
+    const int mode = may_graph && !dwhip_prof_on() && niter > 1 ? dwhip_graph_mode() : 0;
+    if (mode == 0) {
+        for (int k = 0; k < niter; ++k) RCHK(dwhip_job_round(job, job->d.rl || k + 1 < niter, s));
+        return DWHIP_OK;
+    }
+    const int nfull = mode == 2 ? 1 : (job->d.rl ? niter : niter - 1);
+    LoopCtx c{job, niter, mode == 1};
+    hipGraphExec_t exec = nullptr;
+    RCHK(dwhip_capture(loop_body, &c, &exec));
+    *exec_out = exec;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nfull && e == hipSuccess; ++k) e = hipGraphLaunch(exec, s);
+    if (e != hipSuccess) return dwhip_hip_fail(e, "hipGraphLaunch", __LINE__);
+    job->host_iter = mode == 2 ? niter : nfull;
+    if (mode == 1 && !job->d.rl) RCHK(dwhip_job_round(job, false, s));
+    return DWHIP_OK;
+}
+
+// The PSF inserted at the transform size w and shifted so that its
+// maximum (lowest linear index on ties) is at the origin: the volume Z of
+// method_shb.c:184-209 (fim_insert, fim_argmax_max, fim_circshift).
+static void shifted_psf_host(const float* psf, int64_t pM, int64_t pN, int64_t pP, const int64_t* w, float* Z)
+{
+    const size_t np = (size_t)pM * pN * pP;
+    size_t a = 0;
+    for (size_t i = 1; i < np; ++i)
+        if (psf[i] > psf[a]) a = i;
+    const int64_t mz = (int64_t)(a / ((size_t)pM * pN)), my = (int64_t)((a / pM) % pN), mx = (int64_t)(a % pM);
+    for (int64_t z = 0; z < w[2]; ++z)
+        for (int64_t y = 0; y < w[1]; ++y)
+            for (int64_t x = 0; x < w[0]; ++x) {
+                const int64_t sx = (x + mx) % w[0], sy = (y + my) % w[1], sz = (z + mz) % w[2];
+                Z[x + w[0] * (y + w[1] * z)] =
+                    (sx < pM && sy < pN && sz < pP) ? psf[sx + pM * (sy + pN * sz)] : 0.f;
+            }
+}
+
+// The loop with the caller's per-iteration hook: every round launched
+// directly, the host waits for it, reads its error and, every hook_every
+// iterations, the iterate x_k cropped into `out` (its scratch until the
+// final crop) and copied to the host; then the optional full-size calls.
+static int hooked_loop(DwhipJob* job, const dwhip_params* prm, float* out, hipStream_t s, int* nrun)
+{
+    const bool fixed = prm->iter_type == DWHIP_ITER_FIXED;
+    const JobDev& d = job->d;
+    const size_t n_img = (size_t)d.M * d.N * d.P;
+    std::vector<float> hx(prm->hook_every > 0 ? n_img : 0);
+    float error = 1.f, lasterror = 1.f;
+    *nrun = 0;
+    for (int k = 0;; ++k) {
+        if (fixed ? k >= prm->niter : !iterator_continue(prm, k, error, lasterror)) break;
+        RCHK(dwhip_job_round(job, job->d.rl || !fixed || k + 1 < prm->niter, s));
+        *nrun = k + 1;
+        float ek = 0;
+        HIPCHK(hipMemcpyAsync(&ek, d.err + k, sizeof(float), hipMemcpyDeviceToHost, s));
+        const float* xh = nullptr;
+        if (prm->hook_every > 0 && k % prm->hook_every == 0) {
+            // x_k: RL's rounds end with x_k in U[k & 1] as SHB's do
+            HIPCHK(dwhip_launch_crop(job->jd, 0, k, out, 1024, s));
+            HIPCHK(hipMemcpyAsync(hx.data(), out, n_img * sizeof(float), hipMemcpyDeviceToHost, s));
+            xh = hx.data();
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        prm->hook(prm->hook_user, k, ek, xh, d.M, d.N, d.P);
+        lasterror = error;
+        error = ek;
+    }
+    if (prm->hook_full) {
+        const int64_t w[3] = {d.nx, d.ny, d.nz};
+        const size_t V = (size_t)w[0] * w[1] * w[2];
+        std::vector<float> full(V);
+        if (job->d.rl && *nrun > 0) RCHK(dwhip_job_final_update(job, job->bd, job->jd, s));
+        const int k = dwhip_result_iterate(job->d.rl != 0, *nrun);
+        const float* U = (k & 1) ? d.U1 : d.U0;
+        HIPCHK(hipMemcpyAsync(full.data(), U, V * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        prm->hook(prm->hook_user, -1, 0.f, full.data(), w[0], w[1], w[2]);
+        std::vector<float> hp((size_t)d.pM * d.pN * d.pP);
+        HIPCHK(hipMemcpy(hp.data(), d.psf, hp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        shifted_psf_host(hp.data(), d.pM, d.pN, d.pP, w, full.data());
+        prm->hook(prm->hook_user, -2, 0.f, full.data(), w[0], w[1], w[2]);
+    }
+    return DWHIP_OK;
+}
+
 static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
-                   int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, hipStream_t s)
+                   int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, hipStream_t s, void* mem,
+                   size_t cap)
 {
     DwhipJob job;
     const float* start = (prm->start != DWHIP_START_FLAT) ? prm->start_image : nullptr;
-    int rc = dwhip_job_create(&job, im, M, N, P, psf, pM, pN, pP, prm, start);
-    if (rc != DWHIP_OK) {
-        dwhip_job_free(&job);
-        return rc;
+    int rc;
+    if (!mem && prm->planning == DWHIP_PLAN_MEASURE) {
+        // measure this transform size's plans on its first use (a job bound
+        // to a caller's block allocates nothing, so plans by rule or cache)
+        int64_t w[3], ref[3];
+        if (dwhip_work_size(M, N, P, pM, pN, pP, prm->border_quality, w, ref) == DWHIP_OK) {
+            for (int d = 0; d < 3; ++d)
+                if (prm->work[d] > 0) w[d] = prm->work[d];
+            rc = dwhip_plan_ensure(w, prm->method);
+            if (rc != DWHIP_OK) return rc;
+        }
     }
-    hipEvent_t e0, e1, e2;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventCreate(&e2);
-    hipEventRecord(e0, s);
+    if (mem) {
+        rc = dwhip_job_plan(&job, im, M, N, P, psf, pM, pN, pP, prm, start);
+        if (rc == DWHIP_OK && job.bytes > cap) rc = DWHIP_ERR_NOMEM;
+        if (rc == DWHIP_OK) rc = dwhip_job_bind(&job, mem, s);
+        if (rc != DWHIP_OK) {
+            job.mem = nullptr;  // not ours to free
+            return rc;
+        }
+    } else {
+        rc = dwhip_job_create(&job, im, M, N, P, psf, pM, pN, pP, prm, start);
+        if (rc != DWHIP_OK) {
+            dwhip_job_free(&job);
+            return rc;
+        }
+    }
+    const bool timed = mem == nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (timed) {
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventCreate(&e2);
+        hipEventRecord(e0, s);
+    }
     rc = dwhip_job_setup(&job, s);
-    hipEventRecord(e1, s);
+    if (timed) hipEventRecord(e1, s);
 
     const bool fixed = prm->iter_type == DWHIP_ITER_FIXED;
     int nrun = 0;
-    std::vector<float> errs(job.d.niter + 1, 0.f);
-    if (rc == DWHIP_OK) {
+    hipGraphExec_t loop_graph = nullptr;
+    if (rc == DWHIP_OK && prm->hook) {
+        rc = hooked_loop(&job, prm, out, s, &nrun);
+    } else if (rc == DWHIP_OK) {
         if (fixed) {
-            for (int k = 0; k < prm->niter && rc == DWHIP_OK; ++k) {
-                // SHB returns x_{n-1}: the last adjoint half feeds nothing
-                rc = dwhip_job_round(&job, job.d.rl || k + 1 < prm->niter, s);
-                nrun = k + 1;
-            }
+            rc = fixed_loop(&job, prm->niter, mem == nullptr, s, &loop_graph);
+            if (rc == DWHIP_OK) nrun = prm->niter;
         } else {
             // the error of iteration k decides whether k+1 runs
             float error = 1.f, lasterror = 1.f;
@@ -646,8 +954,10 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
         }
     }
     // RL returns x_n: one more update pass turns the last adjoint into x_n
-    if (rc == DWHIP_OK && nrun > 0 && job.d.rl) rc = dwhip_job_final_update(&job, job.bd, job.jd, s);
-    hipEventRecord(e2, s);
+    // (the hooked loop has run it when it handed x_n out at full size)
+    if (rc == DWHIP_OK && nrun > 0 && job.d.rl && !(prm->hook && prm->hook_full))
+        rc = dwhip_job_final_update(&job, job.bd, job.jd, s);
+    if (timed) hipEventRecord(e2, s);
     if (rc == DWHIP_OK) {
         // with no iteration run (REL stopping before iteration 0) the result
         // is the start guess x_0, which the swap-back of method_shb.c:385-390
@@ -657,6 +967,7 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
     }
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess && rc == DWHIP_OK) rc = dwhip_hip_fail(e, "synchronize", __LINE__);
+    if (loop_graph) (void)hipGraphExecDestroy(loop_graph);
     if (dwhip_prof_on()) dwhip_prof_collect();
     if (rc == DWHIP_OK && st) {
         JobState hs;
@@ -667,11 +978,14 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
             st->work[d] = job.work[d];
             st->ref_work[d] = job.ref_work[d];
         }
-        float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
-        st->setup_ms = ms;
-        hipEventElapsedTime(&ms, e1, e2);
-        st->iter_ms = ms;
+        st->setup_ms = st->iter_ms = 0;
+        if (timed) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            st->setup_ms = ms;
+            hipEventElapsedTime(&ms, e1, e2);
+            st->iter_ms = ms;
+        }
         if (st->err_trace && st->err_trace_len > 0 && nrun > 0) {
             const int n = std::min(nrun, st->err_trace_len);
             hipMemcpy(st->err_trace, job.d.err, n * sizeof(float), hipMemcpyDeviceToHost);
@@ -684,10 +998,12 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
                    (long long)job.ref_work[0], (long long)job.ref_work[1], (long long)job.ref_work[2], nrun,
                    st->setup_ms, st->iter_ms);
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipEventDestroy(e2);
-    dwhip_job_free(&job);
+    if (timed) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipEventDestroy(e2);
+        dwhip_job_free(&job);
+    }
     return rc;
 }
 
@@ -705,8 +1021,9 @@ static int check_args(const float* im, int64_t M, int64_t N, int64_t P, const fl
     return DWHIP_OK;
 }
 
-static int run_device(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
-                         int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, void* stream)
+int dwhip_run_device_in(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM,
+                        int64_t pN, int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st,
+                        void* stream, void* mem, size_t cap)
 {
     int rc = check_args(im, M, N, P, psf, pM, pN, pP, prm, out);
     if (rc != DWHIP_OK) return rc;
@@ -723,7 +1040,13 @@ static int run_device(const float* im, int64_t M, int64_t N, int64_t P, const fl
         }
         return DWHIP_OK;
     }
-    return run_job(im, M, N, P, psf, pM, pN, pP, prm, out, st, s);
+    return run_job(im, M, N, P, psf, pM, pN, pP, prm, out, st, s, mem, cap);
+}
+
+static int run_device(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
+                      int64_t pP, const dwhip_params* prm, float* out, dwhip_stats* st, void* stream)
+{
+    return dwhip_run_device_in(im, M, N, P, psf, pM, pN, pP, prm, out, st, stream, nullptr, 0);
 }
 
 static int run_host(const float* im, int64_t M, int64_t N, int64_t P, const float* psf, int64_t pM, int64_t pN,
@@ -758,7 +1081,7 @@ static int run_host(const float* im, int64_t M, int64_t N, int64_t P, const floa
     } else {
         dwhip_params p2 = *prm;
         p2.start_image = dstart;
-        rc = run_job(dim, M, N, P, dpsf, pM, pN, pP, &p2, dout, st, s);
+        rc = run_job(dim, M, N, P, dpsf, pM, pN, pP, &p2, dout, st, s, nullptr, 0);
         if (rc == DWHIP_OK) {
             e = hipMemcpy(out, dout, n * sizeof(float), hipMemcpyDeviceToHost);
             if (e != hipSuccess) rc = dwhip_hip_fail(e, "download", __LINE__);
@@ -818,7 +1141,7 @@ int dwhip_fft3_device(float* vol, float* spec, int64_t nx, int64_t ny, int64_t n
 {
     if (!vol || !spec) return DWHIP_ERR_ARG;
     int64_t w[3] = {nx, ny, nz};
-    if (!size_ok(w)) return DWHIP_ERR_SIZE;
+    if (!dwhip_fast_size_ok(w)) return DWHIP_ERR_SIZE;
     int rc = dwhip_init();
     if (rc != DWHIP_OK) return rc;
     hipStream_t s = (hipStream_t)stream;

@@ -1,0 +1,45 @@
+// Device descriptors of the any-size path: deconwolf's SHB and RL loops on
+// full complex volumes, for work sizes the fused passes cannot transform
+// (border quality 0 and 1 at deconwolf's exact size, whose extents may hold
+// any prime factor or be odd).  Each axis is a complex FFT of its length n:
+// Stockham passes in LDS with radices 2, 3, 4, 5 and 7 when n factors into
+// them, and Bluestein's chirp-z form otherwise (a length-m FFT pair with
+// m >= 2n - 1 that does factor).
+#pragma once
+#include <stdint.h>
+
+#include "shb_dev.h"
+
+#define GEN_MAX_STAGES 16
+
+struct GenPlanDev {
+    int n;           // transform length along the axis
+    int m;           // FFT length run in LDS (n, or the Bluestein length)
+    int bluestein;   // 1: chirp-z through a length-m FFT pair
+    int B;           // pencils per workgroup
+    int nst;         // stages of the length-m FFT
+    int radix[GEN_MAX_STAGES];
+    int ns[GEN_MAX_STAGES];     // product of the radices before the stage
+    int twoff[GEN_MAX_STAGES];  // twiddles of the stage (forward sign) in tw
+    int chirp;       // n values c_j = exp(-i pi j^2 / n) in tw (Bluestein)
+    int hf, hi;      // m values each: FFT_m(conj(c) / m) and FFT_m(c / m), circularly extended
+};
+
+struct GenDev {
+    int nx, ny, nz;
+    int M, N, P;
+    int pM, pN, pP;
+    int rl, positivity, metric, has_w;
+    float alphamax;
+    float2* Z;        // complex work volume [nz][ny][nx]
+    float2* K;        // OTF / (nx*ny*nz), [nz][ny][nx]
+    float* U0;        // x_k lives in U[k & 1], nx*ny*nz
+    float* U1;
+    float* W;         // Bertero weights (has_w)
+    const float* im;  // M*N*P image
+    const float* psf; // pM*pN*pP PSF
+    double* part;     // one partial sum per workgroup of the ratio kernel
+    JobState* st;
+    const float2* tw; // twiddles, chirps and Bluestein filters
+    GenPlanDev plan[3];
+};

@@ -1,0 +1,435 @@
+// Kernels of the any-size path (generic_dev.h): per-axis complex FFTs of
+// any length in LDS (Stockham radix 2/3/4/5/7, Bluestein for other
+// factors) and the elementwise steps of deconwolf's SHB and RL iterations
+// on full complex volumes.  The arithmetic of the elementwise steps is the
+// same, operation for operation, as the fused passes' (shb_passes.h), which
+// restate src/method_shb.c:22-70,333-376 and src/method_rl.c:45-97,357-366.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "generic_dev.h"
+
+#define GTHR 256
+
+namespace {
+
+__device__ __forceinline__ float2 gmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 gmulc(float2 a, float2 b)  // a * conj(b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 gconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// alpha of method_shb.c:328-330 and the extrapolation of :333-339, as in
+// the fused update pass
+__device__ __forceinline__ double g_alpha(int iter, float alphamax)
+{
+    double a = ((float)iter - 1.0) / ((float)iter + 2.0);
+    if (a < 0) a = 0;
+    if (a > alphamax) a = alphamax;
+    return a;
+}
+__device__ __forceinline__ float g_extrap(float x, float xp, double alpha, float bg)
+{
+    float p = (float)((double)x + alpha * (double)(x - xp));
+    return p < bg ? bg : p;
+}
+
+// cos and sin of 2 pi q / R for R in {3, 5, 7}
+template <int R>
+__device__ __forceinline__ void groot(int q, float* c, float* s)
+{
+    const int h = q <= R / 2 ? q : R - q;
+    const float sg = q <= R / 2 ? 1.f : -1.f;
+    float cc = 1.f, ss = 0.f;
+    if (R == 3) {
+        if (h == 1) { cc = -0.5f; ss = 0.8660254037844386f; }
+    } else if (R == 5) {
+        if (h == 1) { cc = 0.30901699437494745f; ss = 0.9510565162951535f; }
+        if (h == 2) { cc = -0.8090169943749475f; ss = 0.5877852522924731f; }
+    } else if (R == 7) {
+        if (h == 1) { cc = 0.6234898018587336f; ss = 0.7818314824680298f; }
+        if (h == 2) { cc = -0.22252093395631434f; ss = 0.9749279121818236f; }
+        if (h == 3) { cc = -0.9009688679024191f; ss = 0.4338837391175582f; }
+    }
+    *c = cc;
+    *s = sg * ss;
+}
+
+// y_q = sum_t v_t exp(sign 2 pi i t q / R) for R in {2, 3, 4, 5, 7}
+template <int R>
+__device__ __forceinline__ void gdft(float2* v, int sign)
+{
+    if (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = make_float2(a.x + b.x, a.y + b.y);
+        v[1] = make_float2(a.x - b.x, a.y - b.y);
+    } else if (R == 4) {
+        const float2 a = v[0], b = v[1], c = v[2], d = v[3];
+        const float2 s02 = make_float2(a.x + c.x, a.y + c.y), d02 = make_float2(a.x - c.x, a.y - c.y);
+        const float2 s13 = make_float2(b.x + d.x, b.y + d.y), d13 = make_float2(b.x - d.x, b.y - d.y);
+        // (sign i) * d13
+        const float2 r = sign < 0 ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
+        v[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+        v[1] = make_float2(d02.x + r.x, d02.y + r.y);
+        v[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+        v[3] = make_float2(d02.x - r.x, d02.y - r.y);
+    } else {
+        float cs[R], sn[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            groot<R>(q, &cs[q], &sn[q]);
+            sn[q] *= (float)sign;
+        }
+        float2 y[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int t = 1; t < R; ++t) {
+                const int e = (t * q) % R;
+                acc.x += v[t].x * cs[e] - v[t].y * sn[e];
+                acc.y += v[t].x * sn[e] + v[t].y * cs[e];
+            }
+            y[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = y[q];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void gstage(const float2* __restrict__ src, float2* __restrict__ dst, int m, int nb,
+                                       int ns, const float2* __restrict__ tw, int sign)
+{
+    const int L = m / R;
+    for (int j = threadIdx.x; j < nb * L; j += GTHR) {
+        const int b = j / L, jj = j - b * L, k = jj % ns;
+        float2 v[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = src[b * m + jj + t * L];
+        if (ns > 1) {
+#pragma unroll
+            for (int t = 1; t < R; ++t) {
+                const float2 w = tw[k * (R - 1) + t - 1];
+                v[t] = sign < 0 ? gmul(v[t], w) : gmulc(v[t], w);
+            }
+        }
+        gdft<R>(v, sign);
+        const int o = b * m + (jj - k) * R + k;
+#pragma unroll
+        for (int t = 0; t < R; ++t) dst[o + t * ns] = v[t];
+    }
+}
+
+// Unnormalised length-m FFT of nb pencils (pencil b at a + b*m), ping-pong
+// between a and t; returns the buffer holding the result.
+__device__ float2* gfft(float2* a, float2* t, const GenPlanDev& pl, const float2* __restrict__ tw, int nb, int sign)
+{
+    float2 *src = a, *dst = t;
+    for (int s = 0; s < pl.nst; ++s) {
+        const float2* tws = tw + pl.twoff[s];
+        switch (pl.radix[s]) {
+        case 2: gstage<2>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
+        case 3: gstage<3>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
+        case 4: gstage<4>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
+        case 5: gstage<5>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
+        default: gstage<7>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
+        }
+        __syncthreads();
+        float2* x = src;
+        src = dst;
+        dst = x;
+    }
+    return src;
+}
+
+// Pencil geometry of tile t along AXIS: base offset of pencil 0, offset
+// between pencils, element stride, pencils in the tile.
+template <int AXIS>
+__device__ __forceinline__ void gpencils(const GenDev& G, int B, int t, size_t* base, size_t* pstep, size_t* estep,
+                                         int* nb)
+{
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    if (AXIS == 0) {
+        const int r0 = t * B;
+        *base = (size_t)r0 * nx;
+        *pstep = nx;
+        *estep = 1;
+        *nb = min(B, ny * nz - r0);
+    } else {
+        const int nxb = (nx + B - 1) / B;
+        const int xb = t % nxb, o = t / nxb;  // o: z for AXIS 1, y for AXIS 2
+        const int x0 = xb * B;
+        *base = AXIS == 1 ? (size_t)x0 + (size_t)nx * ny * o : (size_t)x0 + (size_t)nx * o;
+        *pstep = 1;
+        *estep = AXIS == 1 ? (size_t)nx : (size_t)nx * ny;
+        *nb = min(B, nx - x0);
+    }
+}
+
+template <int AXIS>
+__device__ __forceinline__ int gntiles(const GenDev& G, int B)
+{
+    if (AXIS == 0) return (G.ny * G.nz + B - 1) / B;
+    const int nxb = (G.nx + B - 1) / B;
+    return nxb * (AXIS == 1 ? G.nz : G.ny);
+}
+
+// In-place complex FFT of every pencil of Z along AXIS; sign -1 forward,
+// +1 inverse (unnormalised).
+template <int AXIS>
+__global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, int sign)
+{
+    extern __shared__ float2 sh[];
+    const GenDev& G = *gd;
+    const GenPlanDev& pl = G.plan[AXIS];
+    const int n = pl.n, m = pl.m, B = pl.B;
+    float2* a = sh;
+    float2* t = sh + (size_t)B * m;
+    const float2* __restrict__ tw = G.tw;
+    const float2* chirp = tw + pl.chirp;
+    const float2* H = tw + (sign < 0 ? pl.hf : pl.hi);
+    float2* __restrict__ Z = G.Z;
+    const int ntile = gntiles<AXIS>(G, B);
+    for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        size_t base, pstep, estep;
+        int nb;
+        gpencils<AXIS>(G, B, tile, &base, &pstep, &estep, &nb);
+        for (int idx = threadIdx.x; idx < B * m; idx += GTHR) {
+            // x rows: element-fastest (consecutive addresses); y/z columns:
+            // pencil-fastest (adjacent x, consecutive addresses)
+            const int b = AXIS == 0 ? idx / m : idx % B;
+            const int i = AXIS == 0 ? idx - b * m : idx / B;
+            float2 v = make_float2(0.f, 0.f);
+            if (b < nb && i < n) {
+                v = Z[base + b * pstep + i * estep];
+                if (pl.bluestein) v = sign < 0 ? gmul(v, chirp[i]) : gmulc(v, chirp[i]);
+            }
+            a[b * m + i] = v;
+        }
+        __syncthreads();
+        float2* r = gfft(a, t, pl, tw, nb, pl.bluestein ? -1 : sign);
+        if (pl.bluestein) {
+            for (int idx = threadIdx.x; idx < nb * m; idx += GTHR) {
+                const int k = idx % m;
+                r[idx] = gmul(r[idx], H[k]);
+            }
+            __syncthreads();
+            r = gfft(r, r == a ? t : a, pl, tw, nb, +1);
+        }
+        for (int idx = threadIdx.x; idx < B * n; idx += GTHR) {
+            const int b = AXIS == 0 ? idx / n : idx % B;
+            const int i = AXIS == 0 ? idx - b * n : idx / B;
+            if (b >= nb) continue;
+            float2 v = r[b * m + i];
+            if (pl.bluestein) v = sign < 0 ? gmul(v, chirp[i]) : gmulc(v, chirp[i]);
+            Z[base + b * pstep + i * estep] = v;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ size_t gvol(const GenDev& G) { return (size_t)G.nx * G.ny * G.nz; }
+
+// OTF set-up (method_shb.c:184-212): the PSF inserted at the corner and
+// shifted so its maximum (st->mid) is at the origin.
+__global__ __launch_bounds__(GTHR) void gk_psf(const GenDev* __restrict__ gd)
+{
+    const GenDev& G = *gd;
+    const int* mid = G.st->mid;
+    const size_t V = gvol(G);
+    for (size_t i = (size_t)blockIdx.x * GTHR + threadIdx.x; i < V; i += (size_t)gridDim.x * GTHR) {
+        const int x = (int)(i % G.nx);
+        const size_t r = i / G.nx;
+        const int y = (int)(r % G.ny), z = (int)(r / G.ny);
+        const int sx = (x + mid[0]) % G.nx, sy = (y + mid[1]) % G.ny, sz = (z + mid[2]) % G.nz;
+        float v = 0.f;
+        if (sx < G.pM && sy < G.pN && sz < G.pP) v = G.psf[sx + (size_t)G.pM * (sy + (size_t)G.pN * sz)];
+        G.Z[i] = make_float2(v, 0.f);
+    }
+}
+
+// K = FFT(psf) / (float)nx*ny*nz (the divide of fft.c:856 folded into the OTF)
+__global__ __launch_bounds__(GTHR) void gk_otf(const GenDev* __restrict__ gd)
+{
+    const GenDev& G = *gd;
+    const float scale = ((float)G.nx * (float)G.ny) * (float)G.nz;
+    const size_t V = gvol(G);
+    for (size_t i = (size_t)blockIdx.x * GTHR + threadIdx.x; i < V; i += (size_t)gridDim.x * GTHR) {
+        const float2 v = G.Z[i];
+        G.K[i] = make_float2(v.x / scale, v.y / scale);
+    }
+}
+
+// the image box as ones (Bertero weights, method_shb.c:216-253)
+__global__ __launch_bounds__(GTHR) void gk_ones(const GenDev* __restrict__ gd)
+{
+    const GenDev& G = *gd;
+    const size_t V = gvol(G);
+    for (size_t i = (size_t)blockIdx.x * GTHR + threadIdx.x; i < V; i += (size_t)gridDim.x * GTHR) {
+        const int x = (int)(i % G.nx);
+        const size_t r = i / G.nx;
+        const int y = (int)(r % G.ny), z = (int)(r / G.ny);
+        G.Z[i] = make_float2(x < G.M && y < G.N && z < G.P ? 1.f : 0.f, 0.f);
+    }
+}
+
+// W = P1 > 0.01 ? 1/P1 : 0 (method_shb.c:225-234)
+__global__ __launch_bounds__(GTHR) void gk_weights(const GenDev* __restrict__ gd)
+{
+    const GenDev& G = *gd;
+    const size_t V = gvol(G);
+    for (size_t i = (size_t)blockIdx.x * GTHR + threadIdx.x; i < V; i += (size_t)gridDim.x * GTHR) {
+        const float p1 = G.Z[i].x;
+        G.W[i] = p1 > 0.01f ? (float)(1.0 / p1) : 0.f;
+    }
+}
+
+// Z *= K (fft_mul_inplace, fft.c:437-453) or Z *= conj(K) (fft_mul_conj_inplace, fft.c:490-512)
+__global__ __launch_bounds__(GTHR) void gk_mulk(const GenDev* __restrict__ gd, int conj)
+{
+    const GenDev& G = *gd;
+    const size_t V = gvol(G);
+    for (size_t i = (size_t)blockIdx.x * GTHR + threadIdx.x; i < V; i += (size_t)gridDim.x * GTHR)
+        G.Z[i] = conj ? gmulc(G.Z[i], G.K[i]) : gmul(G.Z[i], G.K[i]);
+}
+
+// Update of iteration k (as the fused pass A): SHB finishes iteration k-1,
+// x_k = x~ (p_{k-1} W) clamped to bg with positivity, then extrapolates
+// p_k = max(x_k + a_k (x_k - x_{k-1}), bg) into Z; RL sets x_k = x~ (x_{k-1}
+// W), clamped to bg when bg > 0, into Z.  Iteration 0 takes x_0 (U0 = U1 =
+// start guess).
+__global__ __launch_bounds__(GTHR) void gk_update(const GenDev* __restrict__ gd)
+{
+    const GenDev& G = *gd;
+    const int iter = G.st->iter;
+    const float bg = G.st->bg;
+    float* __restrict__ Uk = (iter & 1) ? G.U1 : G.U0;
+    const float* __restrict__ Ukm1 = (iter & 1) ? G.U0 : G.U1;
+    const double am1 = g_alpha(iter - 1, G.alphamax), al = g_alpha(iter, G.alphamax);
+    const size_t V = gvol(G);
+    for (size_t i = (size_t)blockIdx.x * GTHR + threadIdx.x; i < V; i += (size_t)gridDim.x * GTHR) {
+        const float xkm1 = Ukm1[i];
+        float xk, zv;
+        if (G.rl) {
+            xk = xkm1;
+            if (iter > 0) {
+                const float xt = G.Z[i].x;
+                xk = G.has_w ? xt * (xkm1 * G.W[i]) : xt * xkm1;
+                if (bg > 0) xk = xk < bg ? bg : xk;
+                Uk[i] = xk;
+            }
+            zv = xk;
+        } else {
+            if (iter == 0) {
+                xk = Uk[i];
+            } else {
+                const float xt = G.Z[i].x;
+                const float p = g_extrap(xkm1, Uk[i], am1, bg);
+                xk = G.has_w ? xt * (p * G.W[i]) : xt * p;
+                if (G.positivity) xk = xk < bg ? bg : xk;
+                Uk[i] = xk;
+            }
+            zv = g_extrap(xk, xkm1, al, bg);
+        }
+        G.Z[i] = make_float2(zv, 0.f);
+    }
+}
+
+// Error terms of y = Re(Z) against the image (getError, dw.c:1436-1524)
+// and the ratio into Z: SHB im/y with the copysign rule inside the image, 0
+// outside (method_shb.c:27-48); RL im/y, bg where y <= 0, 1e-6 outside
+// (method_rl.c:45-65).  One partial sum per workgroup (fixed grid).
+__global__ __launch_bounds__(GTHR) void gk_ratio(const GenDev* __restrict__ gd, int metric_idiv)
+{
+    __shared__ double red[GTHR / 64];
+    const GenDev& G = *gd;
+    const float bg = G.st->bg;
+    const size_t V = gvol(G);
+    const float outside = G.rl ? 1e-6f : 0.f;
+    double acc = 0;
+    for (size_t i = (size_t)blockIdx.x * GTHR + threadIdx.x; i < V; i += (size_t)gridDim.x * GTHR) {
+        const int x = (int)(i % G.nx);
+        const size_t rr = i / G.nx;
+        const int yy = (int)(rr % G.ny), zz = (int)(rr / G.ny);
+        float r = outside;
+        if (x < G.M && yy < G.N && zz < G.P) {
+            const float y = G.Z[i].x;
+            const float gv = G.im[x + (size_t)G.M * (yy + (size_t)G.N * zz)];
+            const double yd = y, gdd = gv;
+            const float q = gv / y;
+            if (metric_idiv) {
+                if (yd > 0 && gdd > 0) acc += gdd * (double)logf(q) - (gdd - yd);
+            } else {
+                acc += (yd - gdd) * (yd - gdd);
+            }
+            if (G.rl)
+                r = y > 0 ? q : bg;
+            else if (fabs(yd) < 1e-6)
+                r = gv / (float)copysign(1e-6, yd);
+            else
+                r = q;
+        }
+        G.Z[i] = make_float2(r, 0.f);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < GTHR / 64; ++w) s += red[w];
+        G.part[blockIdx.x] = s;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid, size_t lds, hipStream_t s)
+{
+    switch (axis) {
+    case 0: hipLaunchKernelGGL(gk_axis<0>, dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    case 1: hipLaunchKernelGGL(gk_axis<1>, dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    default: hipLaunchKernelGGL(gk_axis<2>, dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    }
+    return hipGetLastError();
+}
+
+// Raises the dynamic-LDS limit of the axis kernels (once per process).
+hipError_t dwhip_gen_lds_limit(size_t bytes)
+{
+    hipError_t e = hipFuncSetAttribute((const void*)gk_axis<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)gk_axis<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)gk_axis<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return e;
+}
+
+// elementwise steps: 0 psf, 1 otf, 2 ones, 3 weights, 4 mul K, 5 mul conj K,
+// 6 update, 7 ratio (IDIV), 8 ratio (MSE)
+hipError_t dwhip_gen_launch_elem(const GenDev* gd, int op, int grid, hipStream_t s)
+{
+    switch (op) {
+    case 0: hipLaunchKernelGGL(gk_psf, dim3(grid), dim3(GTHR), 0, s, gd); break;
+    case 1: hipLaunchKernelGGL(gk_otf, dim3(grid), dim3(GTHR), 0, s, gd); break;
+    case 2: hipLaunchKernelGGL(gk_ones, dim3(grid), dim3(GTHR), 0, s, gd); break;
+    case 3: hipLaunchKernelGGL(gk_weights, dim3(grid), dim3(GTHR), 0, s, gd); break;
+    case 4: hipLaunchKernelGGL(gk_mulk, dim3(grid), dim3(GTHR), 0, s, gd, 0); break;
+    case 5: hipLaunchKernelGGL(gk_mulk, dim3(grid), dim3(GTHR), 0, s, gd, 1); break;
+    case 6: hipLaunchKernelGGL(gk_update, dim3(grid), dim3(GTHR), 0, s, gd); break;
+    case 7: hipLaunchKernelGGL(gk_ratio, dim3(grid), dim3(GTHR), 0, s, gd, 1); break;
+    case 8: hipLaunchKernelGGL(gk_ratio, dim3(grid), dim3(GTHR), 0, s, gd, 0); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -106,11 +106,17 @@ extern "C" int dwhip_tiles_blend_device(const dwhip_tile* tiles, const float* co
         for (int k = 0; k < 3; ++k) h[i].xsize[k] = (int)tiles[i].xsize[k];
         h[i].out = outs[i];
     }
+    // the tile table from the stream-ordered pool (no device-wide
+    // allocation after the first call); the synchronisation keeps h alive
+    // until its upload has run
     TileDev* d = nullptr;
-    hipError_t e = hipMalloc((void**)&d, sizeof(TileDev) * ntiles);
+    hipError_t e = hipMallocAsync((void**)&d, sizeof(TileDev) * ntiles, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d, h.data(), sizeof(TileDev) * ntiles, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = dwhip_launch_tile_blend(d, ntiles, (int)M, (int)N, (int)P, offset, out, s);
+    if (d) {
+        const hipError_t f = hipFreeAsync(d, s);
+        if (e == hipSuccess) e = f;
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (d) (void)hipFree(d);
     return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "tile blend", __LINE__);
 }

@@ -26,7 +26,8 @@ import numpy as np
 from . import _lib
 from ._lib import (DwhipParams, DwhipStats, check, lib, ITER_ABS, ITER_REL, ITER_FIXED,
                    METRIC_MSE, METRIC_IDIV, START_FLAT, START_LP, START_IDENTITY,
-                   METHOD_RL, METHOD_ID, METHOD_SHB, DWHIP_METHOD_SHB, DWHIP_METHOD_RL)
+                   METHOD_RL, METHOD_ID, METHOD_SHB, DWHIP_METHOD_SHB, DWHIP_METHOD_RL,
+                   FFTW_ESTIMATE, FFTW_MEASURE, PLAN_MEASURE, PLAN_RULE)
 
 
 @dataclass
@@ -51,6 +52,17 @@ class DwOpts:
     outFormat: int = 16               # dw.c:383 (16: uint16 with scaling, 32: float32)
     scaling: float = -1.0             # dw.c:384 (<= 0: automatic)
     verbosity: int = 0
+    # FFTW planner flag (dw.c:400): FFTW_ESTIMATE plans the transforms by
+    # rule, anything else measures a new transform size's plans once
+    fftw3_planning: int = FFTW_MEASURE
+    iterdump: int = 0                 # dw.c:378: the guess every iterdump iterations
+    fulldump: int = 0                 # dw.c:386: the final guess and shifted PSF at transform size
+    # per-iteration callable(iter, error, x) for iterdump/fulldump/tsv-like
+    # uses: x the (P,N,M) guess x_k every `iterdump` iterations (else None);
+    # with fulldump, iter -1 gives the result and -2 the shifted PSF at the
+    # transform size (nz, ny, nx).  Set: every pass runs directly, the host
+    # waits for each iteration.
+    iter_hook: Optional[object] = field(default=None, repr=False)
     # transform size; None = automatic (smallest 2,3,5,7-smooth size >= the
     # reference work size for borderQuality 2)
     work: Optional[Tuple[int, int, int]] = None
@@ -83,7 +95,51 @@ def _params(s: DwOpts) -> DwhipParams:
     if int(s.method) not in (DWHIP_METHOD_SHB, DWHIP_METHOD_RL):
         raise ValueError("method %r has no GPU path (SHB and RL do)" % (s.method,))
     p.method = int(s.method)
+    p.planning = PLAN_RULE if int(s.fftw3_planning) == FFTW_ESTIMATE else PLAN_MEASURE
+    if s.iter_hook is not None:
+        user = s.iter_hook
+
+        def _hook(_u, it, err, x, nx, ny, nz):
+            arr = None
+            if x:
+                arr = np.ctypeslib.as_array(x, shape=(int(nz), int(ny), int(nx))).copy()
+            user(int(it), float(err), arr)
+
+        p.hook = _lib.HOOK(_hook)
+        p.hook_every = int(s.iterdump)
+        p.hook_full = 1 if s.fulldump else 0
+        p._keep = p.hook  # the callback object lives as long as the params
     return p
+
+
+def plan_tune(work, retune=False):
+    """Measures the fused passes' plans for a transform size (as
+    :func:`work_size` returns it) and caches the choice, like FFTW's
+    planner with FFTW_MEASURE; returns (tuned_ms, rule_ms), the summed pass
+    times of the chosen plan and of the rule planner's."""
+    w = (ctypes.c_int64 * 3)(*[int(v) for v in work])
+    tb, tr = ctypes.c_double(), ctypes.c_double()
+    check(lib().dwhip_plan_tune(w, 1 if retune else 0, ctypes.byref(tb), ctypes.byref(tr)), "dwhip_plan_tune")
+    return tb.value, tr.value
+
+
+def plan_get(work):
+    """(measured, radices, tiles) of the plan a job of transform size work
+    uses: radices per axis (x rows of length nx/2, y, z), tiles (x rows, y
+    and z columns per workgroup)."""
+    w = (ctypes.c_int64 * 3)(*[int(v) for v in work])
+    r = (ctypes.c_int * 12)()
+    t = (ctypes.c_int * 3)()
+    rc = lib().dwhip_plan_get(w, r, t)
+    if rc < 0:
+        check(rc, "dwhip_plan_get")
+    rad = [[v for v in r[4 * a:4 * a + 4] if v] for a in range(3)]
+    return bool(rc), rad, tuple(t)
+
+
+def plan_cache_path(path):
+    """Plan cache file ("" for none, None for the default); loads it."""
+    check(lib().dwhip_plan_cache_path(None if path is None else path.encode()), "dwhip_plan_cache_path")
 
 
 def _stats_dict(st: DwhipStats) -> dict:

@@ -23,9 +23,14 @@ class Engine:
     """order: _lib.ORDER_FIFO (arrival order) or ORDER_SHORTEST (least work
     first); active_mvox: cap on the transform voxels (millions) of the jobs
     iterating together, 0 for none; groups: 2 runs the active jobs as two
-    groups on two streams whose passes overlap."""
+    groups on two streams whose passes overlap; rounds_per_step: rounds a
+    step may run while every active job is FIXED (replayed from a captured
+    HIP graph of the round), None for the library's default; aging_ms:
+    with ORDER_SHORTEST, a queued job's work counts as work / (1 + waited /
+    aging_ms)."""
 
-    def __init__(self, hbm_budget_gb=32.0, max_active=64, stream=None, order=0, active_mvox=0, groups=1):
+    def __init__(self, hbm_budget_gb=32.0, max_active=64, stream=None, order=0, active_mvox=0, groups=1,
+                 rounds_per_step=None, aging_ms=0):
         import torch
         cfg = EngineConfig()
         lib().dwhip_engine_config_default(ctypes.byref(cfg))
@@ -34,6 +39,9 @@ class Engine:
         cfg.order = int(order)
         cfg.active_mvox = int(active_mvox)
         cfg.groups = int(groups)
+        if rounds_per_step is not None:
+            cfg.rounds_per_step = int(rounds_per_step)
+        cfg.aging_ms = int(aging_ms)
         h = ctypes.c_void_p()
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().dwhip_engine_create(ctypes.byref(cfg), s, ctypes.byref(h)), "dwhip_engine_create")

@@ -81,10 +81,51 @@ C4 = dict(xy=(256, 384, 512, 768, 1024), z=(20, 40, 60, 80), lambdas=(460.0, 520
           iters=(20, 50, 100), psf=(181, 181, 79), bg=120.0, blob_density=2000.0 / (1024 * 1024 * 60))
 
 
-def describe_stream(n, rate):
-    return ("c4 stream: %d jobs, Poisson arrivals at %g jobs/s, XY uniform over {256,384,512,768,1024}^2, "
+def describe_stream(n, rate, xy=None):
+    xy = xy or C4["xy"]
+    return ("c4 stream: %d jobs, Poisson arrivals at %g jobs/s, XY uniform over {%s}^2, "
             "Z uniform over {20,40,60,80}, Gaussian blobs at the c2 density, background 120, Born-Wolf PSF "
-            "181x181x79 at 460/520/600/690 nm (NA 1.40), iterations uniform over {20,50,100}" % (n, rate))
+            "181x181x79 at 460/520/600/690 nm (NA 1.40), iterations uniform over {20,50,100}"
+            % (n, rate, ",".join(str(v) for v in xy)))
+
+
+def c4_psf(lam, device="cuda:0"):
+    """Born-Wolf PSF of the c4 stream at wavelength lam (nm), from the device
+    generator, normalised like bw_psf."""
+    pM, pN, pP = C4["psf"]
+    raw = psf_bw.bw_psf_device(pN, pP, lam, NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=200.0).cpu().numpy()
+    return fim_normalize_sum1(raw)
+
+
+def make_c4_job(M, P, lam, iters, rng, gen_seed, device="cuda:0", psf=None):
+    """One c4 job of XY size M x M and depth P: Gaussian blobs at the c2
+    density drawn from rng, blurred on the GPU (torch.fft, input generation
+    only), background 120, Poisson noise seeded with gen_seed; the PSF
+    prepared like dw_run (normalise, autocrop, normalise), the offset
+    added to the image."""
+    import torch
+    N = M
+    pM, pN, pP = C4["psf"]
+    psf = c4_psf(lam, device) if psf is None else psf
+    nb = max(1, int(round(C4["blob_density"] * M * N * P)))
+    V = synth.gaussian_blobs((P, N, M), nb, sigma=(1.0, 3.0), amp=(200.0, 4000.0), rng=rng)
+    full = (P + pP - 1, N + pN - 1, M + pM - 1)
+    a = torch.as_tensor(V.astype(np.float32), device=device)
+    b = torch.as_tensor(psf, device=device)
+    B = torch.fft.irfftn(torch.fft.rfftn(a, full) * torch.fft.rfftn(b, full), full)
+    cz, cy, cx = (pP - 1) // 2, (pN - 1) // 2, (pM - 1) // 2
+    B = B[cz:cz + P, cy:cy + N, cx:cx + M].clamp_min(0) + C4["bg"]
+    g = torch.Generator(device=device)
+    g.manual_seed(int(gen_seed))
+    im = torch.poisson(B, generator=g).float().contiguous()
+    del a, B
+    s = DwOpts(nIter=iters, iter_type=2)
+    ps = fim_normalize_sum1(psf)
+    ps = psf_autocrop(ps, (P, N, M), s)
+    ps = fim_normalize_sum1(ps)
+    im += float(s.offset)
+    return dict(im=im, psf=torch.as_tensor(ps, device=device).contiguous(), opts=s, shape=(M, N, P), iters=iters,
+                lam=lam)
 
 
 def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
@@ -92,43 +133,22 @@ def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
     device, prepared like dw_run), opts, arrival (s).  The PSFs come from
     the device Born-Wolf generator; blurring runs on the GPU through
     torch.fft (input generation only)."""
-    import torch
     rng = np.random.default_rng(seed)
     xy = xy or C4["xy"]
     psfs = {}
     jobs = []
     t = 0.0
-    pM, pN, pP = C4["psf"]
     for i in range(n):
         t += rng.exponential(1.0 / rate)
-        M = N = int(rng.choice(xy))
+        M = int(rng.choice(xy))
         P = int(rng.choice(C4["z"]))
         lam = float(rng.choice(C4["lambdas"]))
         iters = int(rng.choice(C4["iters"]))
         if lam not in psfs:
-            # dw_bw's model on the GPU, normalised like bw_psf
-            raw = psf_bw.bw_psf_device(pN, pP, lam, NA=1.40, ni=1.515, dx_nm=65.0, dz_nm=200.0).cpu().numpy()
-            psfs[lam] = fim_normalize_sum1(raw)
-        psf = psfs[lam]
-        nb = max(1, int(round(C4["blob_density"] * M * N * P)))
-        V = synth.gaussian_blobs((P, N, M), nb, sigma=(1.0, 3.0), amp=(200.0, 4000.0), rng=rng)
-        full = (P + pP - 1, N + pN - 1, M + pM - 1)
-        a = torch.as_tensor(V.astype(np.float32), device=device)
-        b = torch.as_tensor(psf, device=device)
-        B = torch.fft.irfftn(torch.fft.rfftn(a, full) * torch.fft.rfftn(b, full), full)
-        cz, cy, cx = (pP - 1) // 2, (pN - 1) // 2, (pM - 1) // 2
-        B = B[cz:cz + P, cy:cy + N, cx:cx + M].clamp_min(0) + C4["bg"]
-        g = torch.Generator(device=device)
-        g.manual_seed(int(seed * 100003 + i))
-        im = torch.poisson(B, generator=g).float().contiguous()
-        del a, B
-        s = DwOpts(nIter=iters, iter_type=2)
-        ps = fim_normalize_sum1(psf)
-        ps = psf_autocrop(ps, (P, N, M), s)
-        ps = fim_normalize_sum1(ps)
-        im += float(s.offset)
-        jobs.append(dict(im=im, psf=torch.as_tensor(ps, device=device).contiguous(), opts=s, arrival=t,
-                         shape=(M, N, P), iters=iters, lam=lam))
+            psfs[lam] = c4_psf(lam, device)
+        j = make_c4_job(M, P, lam, iters, rng, seed * 100003 + i, device=device, psf=psfs[lam])
+        j["arrival"] = t
+        jobs.append(j)
     return jobs
 
 

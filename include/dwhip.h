@@ -73,7 +73,30 @@ typedef struct {
     int verbosity;       /* 0 silent; > 0 prints sizes and timings to stdout */
     int method;          /* DWHIP_METHOD_*: read by the engine and the tiled path;
                             dwhip_shb_run* and dwhip_rl_run* set it themselves */
+    int planning;        /* DWHIP_PLAN_*: fftw3_planning (src/dw.h:201) */
+    /* Per-iteration host hook, for deconwolf's --iterdump, --tsv and --fulldump
+     * (method_shb.c:205-209,302-322,380,407-411); NULL for none.  With a hook every
+     * pass is launched directly and the host waits for each iteration.  After
+     * iteration k (k = 0, 1, ...) hook(hook_user, k, e_k, x, nx, ny, nz) is called
+     * with e_k the iteration's error and x the iterate x_k (the guess at the top of
+     * iteration k), cropped to M*N*P in host memory, when hook_every > 0 and
+     * k % hook_every == 0, else x = NULL.  With hook_full != 0 two calls follow the
+     * loop: k = -1 with the returned iterate at the transform size (work[0..2]
+     * extents) and k = -2 with the PSF inserted at the transform size and shifted
+     * so its maximum is at the origin (the volume deconvolve_shb transforms). */
+    void (*hook)(void *user, int iter, float error, const float *x, int64_t nx, int64_t ny, int64_t nz);
+    void *hook_user;
+    int hook_every;
+    int hook_full;
 } dwhip_params;
+
+/* Transform planning, the analogue of deconwolf's fftw3_planning flag
+ * (FFTW_ESTIMATE / FFTW_MEASURE, src/dw.c:400,1068): RULE plans by rule
+ * (and uses a plan measured before, from the plan cache); MEASURE also
+ * measures the plans of a new transform size on its first use
+ * (dwhip_plan_tune) and caches the result. */
+#define DWHIP_PLAN_RULE 0
+#define DWHIP_PLAN_MEASURE 1
 
 /* Results besides the output volume. */
 typedef struct {
@@ -109,6 +132,28 @@ int dwhip_init(void);
  */
 int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int64_t pP,
                     int border_quality, int64_t *work, int64_t *ref_work);
+
+/*
+ * Measured planning (the analogue of FFTW's planner and wisdom file,
+ * src/fft.c:135-172,206-319).  dwhip_plan_tune times, axis by axis, the
+ * radix orders and tile shapes the kernels can run for the transform size
+ * work (as dwhip_work_size returns it) against the rule planner's plan,
+ * keeps a candidate only when it is more than 3 % faster, and stores the
+ * plan in the process's plan cache and its file; retune != 0 measures a
+ * size already cached.  tuned_ms / rule_ms (may be NULL) receive the
+ * summed pass times of the chosen and of the rule's plan.  The cache file
+ * is $DWHIP_PLAN_CACHE, else $HOME/.config/deconwolf/dwhip_plans_gfx950.txt
+ * (where deconwolf keeps its FFTW wisdom) when $HOME/.config exists;
+ * dwhip_plan_cache_path sets another ("" for none, NULL for the default)
+ * and loads it.  dwhip_plan_get writes the plan a job of size work uses:
+ * radices[12] (x-row, y and z forward radices, 4 each, 0-padded) and
+ * tiles[3] (x rows, y and z columns per workgroup); returns 1 for a
+ * measured plan, 0 for the rule's, or an error.  Sizes the fused passes
+ * cannot run give DWHIP_ERR_SIZE (the any-size path has one plan).
+ */
+int dwhip_plan_tune(const int64_t *work, int retune, double *tuned_ms, double *rule_ms);
+int dwhip_plan_cache_path(const char *path);
+int dwhip_plan_get(const int64_t *work, int *radices, int *tiles);
 
 /* Device bytes one job of these sizes needs. */
 size_t dwhip_job_bytes(const int64_t *work, int64_t M, int64_t N, int64_t P, int64_t pM,
@@ -221,10 +266,11 @@ int dwhip_fft3_device(float *vol, float *spec, int64_t nx, int64_t ny, int64_t n
  * until the job is reported done) wait in a FIFO queue and are admitted
  * while their state fits in the budget and fewer than `max_active` run.
  * Every dwhip_engine_step() admits what fits, sets the new jobs up, runs
- * one SHB iteration of every active job (one launch per pass for all of
- * them), and completes the jobs that reached their iteration count or
- * stopping rule: their result is in `out` when the step returns and their
- * memory is free for the next admissions.  Results equal those of
+ * SHB iterations of every active job (one launch per pass for all of them;
+ * one round, or up to rounds_per_step while every active job is FIXED), and
+ * completes the jobs that reached their iteration count or stopping rule:
+ * their result is in `out` when the step returns and their memory is free
+ * for the next admissions.  Results equal those of
  * dwhip_shb_run_device for the same job, whatever else runs beside it.
  * Not thread-safe: one host thread drives an engine.
  */
@@ -243,7 +289,14 @@ typedef struct {
     int groups;         /* 1 (default) or 2: the active jobs split in two groups of about
                            equal voxels whose rounds run on two streams, so that passes
                            of one group overlap those of the other */
-    int reserved[3];
+    int rounds_per_step; /* rounds one dwhip_engine_step may run (default 8, 0 reads as 1):
+                           while every active job is FIXED, a step runs rounds up to the
+                           next completion, all but the last replayed from one captured
+                           HIP graph of the round, with one host synchronisation */
+    int aging_ms;       /* DWHIP_ORDER_SHORTEST: a queued job's work counts as
+                           work / (1 + waited / aging_ms), so large jobs are not
+                           passed over for ever; 0 = no aging */
+    int reserved[1];
 } dwhip_engine_config;
 
 void dwhip_engine_config_default(dwhip_engine_config *cfg);

@@ -47,9 +47,12 @@ def main():
         "setup_s": t_setup,
         "iter_s": per_iter,
         "iters_timed": n,
-        "sample": "set-up + %d of %d iterations of the %dx%dx%d job (PSF %dx%dx%d), %s, %d threads, "
-                  "extrapolated to the %d-iteration job" % (n, a.full_iters, M, N, P, psf.shape[2], psf.shape[1],
-                                                            psf.shape[0], backend, threads, a.full_iters),
+        "sample": ("the whole %d-iteration %dx%dx%d job (PSF %dx%dx%d, set-up included), %s, %d threads"
+                   % (n, M, N, P, psf.shape[2], psf.shape[1], psf.shape[0], backend, threads))
+                  if n >= a.full_iters else
+                  ("set-up + %d of %d iterations of the %dx%dx%d job (PSF %dx%dx%d), %s, %d threads, "
+                   "extrapolated to the %d-iteration job" % (n, a.full_iters, M, N, P, psf.shape[2], psf.shape[1],
+                                                             psf.shape[0], backend, threads, a.full_iters)),
     }))
 
 

@@ -19,8 +19,10 @@ void oracle_rfft3(const double *in, int64_t nx, int64_t ny, int64_t nz, double *
 /* inverse of the above, unnormalised; `in` is not modified. */
 void oracle_irfft3(const double *in, int64_t nx, int64_t ny, int64_t nz, double *out);
 
-/* Optional external float FFT (fft_ext.c): returns 0 when it ran. */
+/* Optional external FFT (fft_ext.c): each returns 0 when it ran. */
 int oracle_set_fft_backend(int external, int nthreads);
 int oracle_ext_r2c(const float *in, int64_t nx, int64_t ny, int64_t nz, float *out);
 int oracle_ext_c2r(float *in, int64_t nx, int64_t ny, int64_t nz, float *out);
+int oracle_ext_r2c_d(const double *in, int64_t nx, int64_t ny, int64_t nz, double *out);
+int oracle_ext_c2r_d(double *in, int64_t nx, int64_t ny, int64_t nz, double *out);
 #endif

@@ -81,11 +81,14 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def set_fft_backend(external, nthreads=0):
+def set_fft_backend(external, nthreads=0, double=False):
     """external=True: float transforms through MKL's FFTW3 interface when
     libmkl_rt.so loads (needs MKL_THREADING_LAYER=GNU in the environment
-    before the process starts); returns whether it is in use."""
-    return bool(lib().oracle_set_fft_backend(1 if external else 0, int(nthreads)))
+    before the process starts); double=True: the float64 oracle's too.
+    Returns whether what was asked for is in use."""
+    level = (2 if double else 1) if external else 0
+    got = lib().oracle_set_fft_backend(level, int(nthreads))
+    return got == level if external else True
 
 
 def cfft(x, sign=-1):

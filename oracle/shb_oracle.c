@@ -17,6 +17,23 @@
 #include "fft_oracle.h"
 #include "shb_oracle.h"
 
+#include <stdio.h>
+
+/* With ORACLE_PROGRESS=k in the environment, a line on stderr every k
+ * iterations (long full-size runs report as they go). */
+static void oracle_progress(int nrun)
+{
+    static int every = -1;
+    if (every < 0) {
+        const char *v = getenv("ORACLE_PROGRESS");
+        every = v ? atoi(v) : 0;
+    }
+    if (every > 0 && nrun % every == 0) {
+        fprintf(stderr, "oracle: %d iterations\n", nrun);
+        fflush(stderr);
+    }
+}
+
 static double oracle_now(void)
 {
     struct timespec ts;

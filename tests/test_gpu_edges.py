@@ -38,13 +38,21 @@ def test_iteration0_metrics_closed_form():
     print("edges: metrics", flush=True)
     im, k = ec.metric_case()
     mse, idiv, bg = ec.metric_closed_forms(im)
+    # y_0 = IFFT(K FFT(bg)) comes out of float32 transforms as bg within a
+    # few ulp (the oracle's rounds only once per axis pass and hits bg); allow
+    # |y_0 - bg| <= 4 ulp(bg) = d and propagate it: |dMSE| <= 2 mean|g-bg| d + d^2,
+    # |dIDIV| <= mean|1 - g/bg| d, on top of the 2e-6 of the float32 metric.
+    d = 4 * float(np.spacing(np.float32(bg)))
+    gm = im.astype(np.float64)
     s = _fixed(1, metric=dw.METRIC_MSE)
     dw.deconvolve_shb_hip(im, k, s)
     assert s.bg == np.float32(bg)
-    assert abs(s.err_trace[0] - mse) <= 2e-6 * mse
+    tol = 2e-6 * mse + 2 * np.mean(np.abs(gm - bg)) * d + d * d
+    assert abs(s.err_trace[0] - mse) <= tol, (s.err_trace[0], mse, tol)
     s = _fixed(1)
     dw.deconvolve_shb_hip(im, k, s)
-    assert abs(s.err_trace[0] - idiv) <= 2e-6 * idiv
+    tol = 2e-6 * idiv + np.mean(np.abs(1 - gm / bg)) * d
+    assert abs(s.err_trace[0] - idiv) <= tol, (s.err_trace[0], idiv, tol)
 
 
 def _decreasing_case():

@@ -46,20 +46,31 @@ def solo(im, k, s):
     return out.cpu().numpy(), s2
 
 
-@pytest.mark.parametrize("groups", [1, 2])
-def test_batch_equals_alone(groups):
-    print("engine: mixed batch of", len(JOBS), "groups", groups, flush=True)
+def expected_steps(iters, cap):
+    """Steps of an engine whose jobs are all admitted at once and all FIXED:
+    each step runs rounds up to the next completion, at most cap."""
+    t, steps = 0, 0
+    while t < max(iters):
+        t = min(min(i for i in iters if i > t), t + cap)
+        steps += 1
+    return steps
+
+
+@pytest.mark.parametrize("groups,rounds_per_step", [(1, 1), (2, 1), (1, 8), (2, 8), (2, 3)])
+def test_batch_equals_alone(groups, rounds_per_step):
+    print("engine: mixed batch of", len(JOBS), "groups", groups, "rounds per step", rounds_per_step, flush=True)
     jobs = [make_job(sd, sh, ps, n, **o) for sd, sh, ps, n, o in JOBS]
-    eng = Engine(hbm_budget_gb=2.0, max_active=16, groups=groups)
+    eng = Engine(hbm_budget_gb=2.0, max_active=16, groups=groups, rounds_per_step=rounds_per_step)
     for t, (im, k, s) in enumerate(jobs):
         eng.submit(t, im, k, s)
     done = []
-    rounds = 0
+    steps = 0
     while len(done) < len(jobs):
         done += eng.step()
-        rounds += 1
-        assert rounds < 100
-    assert rounds == max(j[3] for j in JOBS)   # all admitted at once, one iteration per round
+        steps += 1
+        assert steps < 100
+    # all admitted at once; several rounds per step up to each completion
+    assert steps == expected_steps([j[3] for j in JOBS], rounds_per_step)
     for t, (im, k, s) in enumerate(jobs):
         ref, s2 = solo(im, k, s)
         out, st = eng.result(t)
@@ -77,7 +88,8 @@ def test_budget_limited_admission_and_staggered_submission(groups):
     w, r, _ = dw.work_size((16, 36, 40), (7, 9, 9))
     V = w[0] * w[1] * w[2]
     # room for about two of these jobs (a job is ~5 V floats)
-    eng = Engine(hbm_budget_gb=2.5 * 5 * V * 4 / (1 << 30), max_active=4, groups=groups)
+    # one round per step, so that the queue builds up while jobs iterate
+    eng = Engine(hbm_budget_gb=2.5 * 5 * V * 4 / (1 << 30), max_active=4, groups=groups, rounds_per_step=1)
     for t in range(2):
         eng.submit(t, *jobs[t])
     seen_active = 0
@@ -156,7 +168,7 @@ def test_shortest_first_and_voxel_cap():
     w, _, _ = dw.work_size((40, 150, 150), (9, 9, 9), 2)
     assert 1_000_000 < w[0] * w[1] * w[2] < 1_500_000
     for cap, most in ((2, 1), (3, 2)):
-        eng = Engine(hbm_budget_gb=2.0, max_active=8, active_mvox=cap)
+        eng = Engine(hbm_budget_gb=2.0, max_active=8, active_mvox=cap, rounds_per_step=1)
         for t, (im, k, s) in enumerate(big):
             eng.submit(t, im, k, s)
         n, seen = 0, 0

@@ -1,0 +1,234 @@
+"""Parity at every benchmark configuration at full size, against the oracle
+run at deconwolf's own work size (method_shb.c:137-153) — the computation
+deconwolf's CPU SHB does — in float32 and float64.  The GPU runs its
+padded transform size: with border quality 2 a band of pM - 1 voxels
+separates the image's far edges, no band voxel is within the PSF's reach
+of both, so further padding leaves the image region unchanged up to
+rounding (test_oracle.py's padding-invariance pin checks it).  The oracle's transforms go through MKL's FFTW3 interface
+(fftwf_* for float32, fftw_* for float64), checked first against the
+oracle's own transforms on a small case.
+
+* c2: 1024x1024x60, PSF 181x181x79, all 50 iterations; float32 and float64;
+  and with border quality 1 at deconwolf's exact size 1115 x 1115 x 100
+  (the any-size path), 3 iterations, float32.
+* c3: the 2048x2048x60 image in 16 tiles of 512 px with 48 px overlap
+  (tiling.c), 3 iterations per tile through the engine, against the
+  oracle's per-tile float32 SHB blended by the tiling oracle.
+* c4: the two extreme job shapes of the stream, 256x256x20 (PSF cropped to
+  39 planes by psf_autocrop) and 1024x1024x80, in one engine, 20
+  iterations (the smallest count of the stream), float32.
+* c5: one channel (690 nm, 1024x1024x81, PSF 181x181x81 at dz 300 nm), all
+  100 iterations, float32 and float64.
+
+Tolerance: relative L2 <= 1e-4 (the bar deconwolf's float SHB output is
+held to); the error traces to 1e-3 relative.  Each test prints its
+figures and, when DWHIP_PARITY_LOG names a file, appends them to it as
+JSON lines (rel-L2 and max relative error per config and oracle).
+"""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import deconwolf_amd as dw
+from deconwolf_amd import tiling, workloads
+from deconwolf_amd.serve import Engine
+from oracle import pyoracle as po
+from oracle import tiling_oracle as to
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TOL = 1e-4
+
+
+def _threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def max_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def _log(rec):
+    rec = dict(rec)
+    print("parity:", json.dumps(rec), flush=True)
+    path = os.environ.get("DWHIP_PARITY_LOG")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps(rec) + "\n")
+
+
+@pytest.fixture(scope="module")
+def mkl():
+    os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
+    if not po.set_fft_backend(True, _threads(), double=True):
+        pytest.skip("MKL's FFTW3 interface is not available on this host")
+    try:
+        # the external transforms agree with the oracle's own on a small case
+        rng = np.random.default_rng(3)
+        im = (rng.random((10, 24, 30)) * 100 + 10).astype(np.float32)
+        k = np.ones((3, 5, 5), np.float32) / 75
+        ext32, _, _, _ = po.shb(im, k, niter=4)
+        ext64, _, _, _ = po.shb(im, k, niter=4, dtype=np.float64)
+        po.set_fft_backend(False)
+        own32, _, _, _ = po.shb(im, k, niter=4)
+        own64, _, _, _ = po.shb(im, k, niter=4, dtype=np.float64)
+        assert rel_l2(ext32, own32) <= 1e-5
+        assert rel_l2(ext64, own64) <= 1e-12
+        assert po.set_fft_backend(True, _threads(), double=True)
+        yield _threads()
+    finally:
+        po.set_fft_backend(False)
+
+
+def _gpu(im, psf, niter):
+    s = dw.DwOpts(nIter=niter, iter_type=dw.ITER_FIXED)
+    t = time.time()
+    out = dw.deconvolve_shb_hip_device(torch.from_numpy(im).to(DEV), torch.from_numpy(psf).to(DEV), s)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), s, time.time() - t
+
+
+def _against_oracles(config, im, psf, niter, dtypes, threads):
+    out, s, t_gpu = _gpu(im, psf, niter)
+    ref = tuple(s.stats["ref_work"])
+    for dt in dtypes:
+        t = time.time()
+        o, e, bg, n = po.shb(im, psf, niter=niter, work=ref, nthreads=threads, dtype=dt)
+        t_cpu = time.time() - t
+        assert n == niter and s.bg == np.float32(bg)
+        r, m = rel_l2(out, o), max_rel(out, o)
+        et = max_rel(s.err_trace, e)
+        _log(dict(config=config, oracle="float32" if dt == np.float32 else "float64", iterations=niter,
+                  image=list(im.shape[::-1]), gpu_work=list(s.stats["work"]), oracle_work=list(ref),
+                  rel_l2=r, max_rel=m, err_trace_max_rel=et, gpu_s=t_gpu, oracle_s=t_cpu, threads=threads))
+        assert r <= TOL, (config, dt, r)
+        np.testing.assert_allclose(s.err_trace, e, rtol=1e-3)
+
+
+def test_c2_all_iterations(mkl):
+    print("c2: 50 iterations, float32 and float64 oracles", flush=True)
+    im, psf, s0, _ = workloads.make("c2")
+    assert s0.nIter == 50
+    _against_oracles("c2", im, psf, 50, (np.float32, np.float64), mkl)
+
+
+def test_c2_border_quality_1_exact_size(mkl):
+    """c2 with --bq 1: deconwolf's exact work size 1115 x 1115 x 100
+    (1115 = 5 * 223), run by the any-size path (Bluestein along x and y),
+    3 iterations against the float32 oracle at the same size."""
+    print("c2 --bq 1: 1115x1115x100 exact, 3 iterations", flush=True)
+    im, psf, _, _ = workloads.make("c2")
+    s = dw.DwOpts(nIter=3, iter_type=dw.ITER_FIXED, borderQuality=1)
+    t = time.time()
+    out = dw.deconvolve_shb_hip_device(torch.from_numpy(im).to(DEV), torch.from_numpy(psf).to(DEV), s)
+    torch.cuda.synchronize()
+    t_gpu = time.time() - t
+    out = out.cpu().numpy()
+    assert tuple(s.stats["work"]) == tuple(s.stats["ref_work"]) == (1115, 1115, 100)
+    t = time.time()
+    o, e, bg, n = po.shb(im, psf, niter=3, border_quality=1, nthreads=mkl)
+    r, m = rel_l2(out, o), max_rel(out, o)
+    _log(dict(config="c2 --bq 1 (any-size path)", oracle="float32", iterations=3, image=list(im.shape[::-1]),
+              gpu_work=list(s.stats["work"]), oracle_work=list(s.stats["ref_work"]), rel_l2=r, max_rel=m,
+              err_trace_max_rel=max_rel(s.err_trace, e), gpu_s=t_gpu, oracle_s=time.time() - t, threads=mkl))
+    assert n == 3 and s.bg == np.float32(bg)
+    assert r <= 1e-5, r
+    np.testing.assert_allclose(s.err_trace, e, rtol=1e-4)
+
+
+def test_c5_channel_all_iterations(mkl):
+    print("c5: one channel (690 nm), 100 iterations, float32 and float64 oracles", flush=True)
+    job = workloads.make_c5(0, channels=(3,), device=DEV)[0]
+    assert job["opts"].nIter == 100
+    im = job["im"].cpu().numpy()
+    psf = job["psf"].cpu().numpy()
+    del job
+    _against_oracles("c5 channel 690 nm", im, psf, 100, (np.float32, np.float64), mkl)
+
+
+def test_c4_extreme_jobs_in_engine(mkl):
+    """The smallest and largest job shapes of the stream run together in one
+    engine; each result against its own oracle run."""
+    print("c4: 256x256x20 and 1024x1024x80 jobs in one engine, 20 iterations", flush=True)
+    rng = np.random.default_rng(44)
+    jobs = [workloads.make_c4_job(256, 20, 460.0, 20, rng, 4401),
+            workloads.make_c4_job(1024, 80, 690.0, 20, rng, 4402)]
+    assert tuple(jobs[0]["psf"].shape) == (39, 181, 181)
+    assert tuple(jobs[1]["psf"].shape)[0] == 79
+    eng = Engine(hbm_budget_gb=48.0, max_active=8)
+    try:
+        for i, j in enumerate(jobs):
+            eng.submit(i, j["im"], j["psf"], j["opts"])
+        done = set()
+        while len(done) < len(jobs):
+            done.update(eng.step())
+        for i, j in enumerate(jobs):
+            out, st = eng.result(i)
+            out = out.cpu().numpy()
+            im = j["im"].cpu().numpy()
+            psf = j["psf"].cpu().numpy()
+            ref = tuple(st["ref_work"])
+            t = time.time()
+            o, e, bg, n = po.shb(im, psf, niter=20, work=ref, nthreads=mkl)
+            r, m = rel_l2(out, o), max_rel(out, o)
+            _log(dict(config="c4 job %dx%dx%d" % j["shape"], oracle="float32", iterations=20,
+                      image=list(j["shape"]), gpu_work=list(st["work"]), oracle_work=list(ref), rel_l2=r,
+                      max_rel=m, err_trace_max_rel=max_rel(st["err_trace"], e), oracle_s=time.time() - t,
+                      threads=mkl, path="engine"))
+            assert n == 20 and st["iterations"] == 20 and np.float32(st["bg"]) == np.float32(bg)
+            assert r <= TOL, (j["shape"], r)
+            np.testing.assert_allclose(st["err_trace"], e, rtol=1e-3)
+    finally:
+        eng.close()
+
+
+def test_c3_tiles(mkl):
+    """The c3 image in its 16 tiles, 3 iterations each, through the engine;
+    the oracle deconvolves each tile at deconwolf's work size for that tile
+    and blends them as deconvolve_tiles does (tiling.c)."""
+    print("c3: 2048x2048x60 in 16 tiles, 3 iterations", flush=True)
+    c = workloads.C3
+    im_d, psf, s = workloads.make_c3()
+    s.nIter = 3
+    eng = Engine(hbm_budget_gb=64.0, max_active=32)
+    try:
+        out, tiles, opts = tiling.deconvolve_tiles(im_d, psf, s, c["tile"], c["overlap"], engine=eng)
+    finally:
+        eng.close()
+    out = out.cpu().numpy()
+    im = im_d.cpu().numpy()
+    del im_d
+    P, N, M = im.shape
+    ref_tiles = to.create(M, N, P, c["tile"], c["overlap"])
+    assert len(ref_tiles) == len(tiles) == 16
+    t = time.time()
+    refs = []
+    for tl, rt in zip(tiles, ref_tiles):
+        x = to.get_tile(rt, im) + np.float32(s.offset)
+        tp = tiling.tile_psf(psf, x.shape, s)
+        _, ref, _ = dw.work_size(x.shape, tp.shape)
+        o, _, _, _ = po.shb(x, tp, niter=3, work=ref, nthreads=mkl)
+        refs.append(to.finish_tile(o, s.offset))
+    want = to.put_tiles(ref_tiles, refs, im.shape)
+    r, m = rel_l2(out, want), max_rel(out, want)
+    _log(dict(config="c3 tiled 2048x2048x60", oracle="float32", iterations=3, tiles=len(tiles),
+              tile_sizes=sorted({tuple(tl.xsize) for tl in tiles}), rel_l2=r, max_rel=m,
+              oracle_s=time.time() - t, threads=mkl, path="engine"))
+    assert r <= TOL, r

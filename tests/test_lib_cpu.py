@@ -63,14 +63,22 @@ def test_work_size_bq2(shape):
     assert all(a <= 1.1 * b + 8 for a, b in zip(w, r))
 
 
-def test_work_size_bq1_bq0_exact_or_refused():
+def test_work_size_bq1_bq0_exact():
+    """bq 0 and 1 run at deconwolf's exact work size whatever its factors:
+    odd x extents and prime factors go to the any-size path (Bluestein);
+    only extents whose transform does not fit a workgroup's LDS (Bluestein
+    length m >= 2n - 1 with 16 m bytes above 150 KiB) are refused."""
     rc, w, r = _ws(43, 26, 11, 9, 7, 7, 1)
     assert r == (43 + 5, 26 + 4, 11 + 4) and w == r and rc == 0
     rc, w, r = _ws(40, 27, 12, 5, 5, 5, 0)
     assert r == (40, 27, 12) and w == r and rc == 0
-    rc, w, r = _ws(41, 27, 12, 5, 5, 5, 0)   # odd x extent: not transformable exactly
-    assert r == (41, 27, 12) and rc == -2
+    rc, w, r = _ws(41, 27, 12, 5, 5, 5, 0)   # odd x extent
+    assert r == (41, 27, 12) and w == r and rc == 0
     rc, w, r = _ws(40, 11, 12, 5, 5, 5, 0)   # 11 is prime
+    assert w == r == (40, 11, 12) and rc == 0
+    rc, w, r = _ws(1024, 1024, 60, 181, 181, 79, 1)   # 1115 = 5 * 223
+    assert w == r == (1115, 1115, 100) and rc == 0
+    rc, w, r = _ws(9973, 16, 8, 3, 3, 3, 0)   # prime beyond the LDS limit
     assert rc == -2
     assert _ws(0, 1, 1, 1, 1, 1, 2)[0] == -1
     assert _ws(1, 1, 1, 1, 1, 1, 3)[0] == -1
@@ -119,14 +127,17 @@ def test_no_gpu_fails_loudly():
 def test_engine_config_validated_before_device_use():
     """dwhip_engine_create checks its configuration before touching a
     device: bad values give DWHIP_ERR_ARG on any host; the defaults are
-    FIFO admission, no voxel cap, one job group, 64 active jobs."""
+    FIFO admission, no voxel cap, one job group, 64 active jobs, up to 8
+    rounds per step, no aging."""
     L = _lib.lib()
     cfg = _lib.EngineConfig()
     L.dwhip_engine_config_default(ctypes.byref(cfg))
     assert (cfg.max_active, cfg.order, cfg.active_mvox, cfg.groups) == (64, _lib.ORDER_FIFO, 0, 1)
+    assert (cfg.rounds_per_step, cfg.aging_ms) == (8, 0)
     assert ctypes.sizeof(_lib.EngineConfig) == 40   # hbm_budget + 7 ints, 8-byte aligned, as in dwhip.h
     h = ctypes.c_void_p()
-    for field, bad in (("groups", 3), ("groups", -1), ("order", 7), ("active_mvox", -5), ("max_active", 0)):
+    for field, bad in (("groups", 3), ("groups", -1), ("order", 7), ("active_mvox", -5), ("max_active", 0),
+                       ("rounds_per_step", -1), ("aging_ms", -3)):
         c = _lib.EngineConfig()
         L.dwhip_engine_config_default(ctypes.byref(c))
         setattr(c, field, bad)
@@ -139,7 +150,7 @@ class View(ctypes.Structure):
         ("err_rel", ctypes.c_float), ("err_abs", ctypes.c_float), ("bg", ctypes.c_float),
         ("bg_auto", ctypes.c_int), ("borderQuality", ctypes.c_int), ("positivity", ctypes.c_int),
         ("alphamax", ctypes.c_float), ("metric", ctypes.c_int), ("start_condition", ctypes.c_int),
-        ("verbosity", ctypes.c_int)]]
+        ("verbosity", ctypes.c_int), ("fftw3_planning", ctypes.c_int)]]
 
 
 @pytest.fixture(scope="module")
@@ -154,9 +165,13 @@ def adapter():
 
 
 def test_adapter_maps_every_field(adapter):
-    v = View(33, 1, 77, 0.05, 2.5, 12.0, 0, 1, 0, 0.5, 0, 2, 0)
+    v = View(33, 1, 77, 0.05, 2.5, 12.0, 0, 1, 0, 0.5, 0, 2, 0, _lib.FFTW_ESTIMATE)
     p = _lib.DwhipParams()
     adapter.dwhip_adapter_params(ctypes.byref(v), ctypes.byref(p))
+    assert p.planning == _lib.PLAN_RULE        # --noplan (dw.c:1068)
+    v.fftw3_planning = _lib.FFTW_MEASURE       # dw_opts_new's default (dw.c:400)
+    adapter.dwhip_adapter_params(ctypes.byref(v), ctypes.byref(p))
+    assert p.planning == _lib.PLAN_MEASURE
     assert (p.niter, p.iter_type, p.maxiter) == (33, 1, 77)
     assert abs(p.err_rel - 0.05) < 1e-7 and p.err_abs == 2.5 and p.bg == 12.0
     assert (p.bg_auto, p.border_quality, p.positivity) == (0, 1, 0)
@@ -182,3 +197,31 @@ def test_adapter_log_lines(adapter, tmp_path):
     # dw_iterator_show's log format (src/dw.c:327-332)
     assert lines == ["Iteration   1/  3, Idiv=1.500000e+00", "Iteration   2/  3, Idiv=2.500000e-01",
                      "Iteration   3/  3, Idiv=1.250000e-01"]
+
+
+def test_plan_cache_file_and_rule_plan(tmp_path):
+    """dwhip_plan_get without a measured plan gives the rule planner's:
+    fewest stages, then the smallest largest radix (c2's transform:
+    625 = 25*25, 1215 = 15*9*9, 140 = 7*5*4) and the largest tiles; a plan
+    line in the cache file (the format dwhip_plan_tune writes) replaces it,
+    and lines whose radices are not a plan the tuner could choose (each
+    radix multiset in one of its two monotone orders) are ignored."""
+    w = (1250, 1215, 140)
+    dw.plan_cache_path("")
+    measured, rad, tiles = dw.plan_get(w)
+    assert not measured and rad == [[25, 25], [15, 9, 9], [7, 5, 4]] and tiles == (10, 16, 32)
+    f = tmp_path / "plans.txt"
+    f.write_text("gfx950 1250 1215 140 x=5,5,25 xrows=5 y=9,9,15 ycols=8 z=4,5,7 zcols=16\n"
+                 "gfx950 256 256 96 x=11,11 xrows=4 y=16,16 ycols=8 z=6,4,4 zcols=16\n"     # 11 not a radix
+                 "gfx942 256 256 96 x=16,8 xrows=4 y=16,16 ycols=8 z=6,4,4 zcols=16\n")     # other device
+    try:
+        dw.plan_cache_path(str(f))
+        measured, rad, tiles = dw.plan_get(w)
+        assert measured and rad == [[5, 5, 25], [9, 9, 15], [4, 5, 7]] and tiles == (5, 8, 16)
+        measured, rad, _ = dw.plan_get((256, 256, 96))
+        assert not measured and rad[0] == [16, 8]
+        f.write_text("gfx950 1250 1215 140 x=5,25,5 xrows=5 y=9,9,15 ycols=8 z=4,5,7 zcols=16\n")
+        dw.plan_cache_path(str(f))
+        assert not dw.plan_get(w)[0]
+    finally:
+        dw.plan_cache_path("")
