@@ -5,7 +5,7 @@ for its plain Richardson-Lucy method `deconvolve_rl` (src/method_rl.c),
 whose loops run as hand-written HIP kernels for gfx950 in libdwhip.so.
 """
 from .method_shb import (DwOpts, deconvolve, deconvolve_rl_hip, deconvolve_rl_hip_device,
-                         deconvolve_shb_hip, deconvolve_shb_hip_device,
+                         deconvolve_shb_hip, deconvolve_shb_hip_device, deconvolve_shb_hip_device_noalloc,
                          fft3_device, fim_normalize_sum1, plan_cache_path, plan_get, plan_tune,
                          psf_autocrop, work_size)
 from ._lib import (ITER_ABS, ITER_REL, ITER_FIXED, METRIC_MSE, METRIC_IDIV, START_FLAT, START_LP,
@@ -13,7 +13,7 @@ from ._lib import (ITER_ABS, ITER_REL, ITER_FIXED, METRIC_MSE, METRIC_IDIV, STAR
 
 __all__ = [
     "DwOpts", "deconvolve", "deconvolve_shb_hip", "deconvolve_shb_hip_device",
-    "deconvolve_rl_hip",
+    "deconvolve_shb_hip_device_noalloc", "deconvolve_rl_hip",
     "deconvolve_rl_hip_device", "fft3_device", "fim_normalize_sum1", "psf_autocrop", "work_size",
     "DwhipError", "plan_tune", "plan_get", "plan_cache_path", "ITER_ABS", "ITER_REL", "ITER_FIXED", "METRIC_MSE", "METRIC_IDIV",
     "START_FLAT", "START_LP", "START_IDENTITY", "METHOD_RL", "METHOD_SHB",

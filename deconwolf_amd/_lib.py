@@ -12,8 +12,10 @@ LIB_PATH = os.path.join(HERE, "libdwhip.so")
 DWHIP_OK = 0
 ERRORS = {
     -1: "DWHIP_ERR_ARG", -2: "DWHIP_ERR_SIZE", -3: "DWHIP_ERR_HIP", -4: "DWHIP_ERR_NOMEM",
-    -5: "DWHIP_ERR_PSF", -6: "DWHIP_ERR_NODEVICE", -7: "DWHIP_ERR_STATE",
+    -5: "DWHIP_ERR_PSF", -6: "DWHIP_ERR_NODEVICE", -7: "DWHIP_ERR_STATE", -8: "DWHIP_ERR_SCRATCH",
 }
+# size of the static device block of dwhip_shb_run_device_noalloc
+SCRATCH_BYTES = 256 << 20
 
 ITER_ABS, ITER_REL, ITER_FIXED = 0, 1, 2
 METRIC_MSE, METRIC_IDIV = 0, 1
@@ -120,6 +122,8 @@ SIGNATURES = [
                              ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats)]),
     ("dwhip_shb_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                                     ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
+    ("dwhip_shb_run_device_noalloc", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                                            ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats), _vp]),
     ("dwhip_rl_run", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                             ctypes.POINTER(DwhipParams), _vp, ctypes.POINTER(DwhipStats)]),
     ("dwhip_rl_run_device", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,

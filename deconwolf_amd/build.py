@@ -50,6 +50,7 @@ def units():
     u.append(("k_misc.hip", [], "k_misc.o"))
     u.append(("k_output.hip", [], "k_output.o"))
     u.append(("k_bw.hip", [], "k_bw.o"))
+    u.append(("noalloc.hip", [], "noalloc.o"))
     u.append(("k_generic.hip", [], "k_generic.o"))
     for src in ("dwhip.cpp", "profile.cpp", "batch.cpp", "tiling.cpp", "graph.cpp", "generic.cpp", "tune.cpp"):
         if os.path.exists(os.path.join(CSRC, src)):

@@ -29,6 +29,7 @@ const char* dwhip_strerror(int code)
     case DWHIP_ERR_PSF: return "the PSF maximum could not be moved to the origin";
     case DWHIP_ERR_NODEVICE: return "no usable GPU device";
     case DWHIP_ERR_STATE: return "call not valid in this state";
+    case DWHIP_ERR_SCRATCH: return "job larger than the library's static scratch block (DWHIP_SCRATCH_BYTES)";
     default: return "unknown error";
     }
 }
@@ -772,7 +773,7 @@ static bool iterator_continue(const dwhip_params* p, int iter, float error, floa
 // state, or nullptr to allocate one here for the call.  With a given block
 // nothing is allocated on the device (no timing events either: setup_ms
 // and iter_ms read 0), and a job needing more than cap bytes is refused
-// with DWHIP_ERR_NOMEM before anything is queued.
+// with DWHIP_ERR_SCRATCH before anything is queued.
 // FIXED iterations of a job on stream s: niter rounds, the last one of SHB
 // without its adjoint half (SHB returns x_{n-1}, so that half feeds
 // nothing).  Graph mode 1 replays one captured full round for every round
@@ -903,7 +904,7 @@ static int run_job(const float* im, int64_t M, int64_t N, int64_t P, const float
     }
     if (mem) {
         rc = dwhip_job_plan(&job, im, M, N, P, psf, pM, pN, pP, prm, start);
-        if (rc == DWHIP_OK && job.bytes > cap) rc = DWHIP_ERR_NOMEM;
+        if (rc == DWHIP_OK && job.bytes > cap) rc = DWHIP_ERR_SCRATCH;
         if (rc == DWHIP_OK) rc = dwhip_job_bind(&job, mem, s);
         if (rc != DWHIP_OK) {
             job.mem = nullptr;  // not ours to free

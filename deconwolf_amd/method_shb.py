@@ -226,6 +226,16 @@ def deconvolve_shb_hip_device(im, psf, s: DwOpts, out=None, start_image=None):
     return _run_device("dwhip_shb_run_device", im, psf, s, out, start_image)
 
 
+def deconvolve_shb_hip_device_noalloc(im, psf, s: DwOpts, out, start_image=None):
+    """:func:`deconvolve_shb_hip_device` through the entry point that
+    allocates no device memory (the job's state in the library's static
+    block of 256 MiB per GPU; larger jobs raise DWHIP_ERR_SCRATCH).  `out`
+    must be given.  Calls that overlap on one GPU race on that block."""
+    if out is None:
+        raise ValueError("the no-allocation entry point needs `out`")
+    return _run_device("dwhip_shb_run_device_noalloc", im, psf, s, out, start_image)
+
+
 def deconvolve_rl_hip_device(im, psf, s: DwOpts, out=None, start_image=None):
     """:func:`deconvolve_rl_hip` for GPU tensors, as
     :func:`deconvolve_shb_hip_device`."""

@@ -31,6 +31,7 @@ extern "C" {
 #define DWHIP_ERR_PSF (-5)       /* PSF maximum not movable to the origin (method_shb.c:199-203) */
 #define DWHIP_ERR_NODEVICE (-6)  /* no usable gfx950 device */
 #define DWHIP_ERR_STATE (-7)     /* call not valid in the object's state */
+#define DWHIP_ERR_SCRATCH (-8)   /* job larger than DWHIP_SCRATCH_BYTES (no-allocation entry) */
 
 /* ---- option values, numerically equal to deconwolf's enums ---------- */
 /* dw_iter_type (src/dw.h:79-83) */
@@ -184,6 +185,22 @@ int dwhip_shb_run(const float *im, int64_t M, int64_t N, int64_t P, const float 
 int dwhip_shb_run_device(const float *im, int64_t M, int64_t N, int64_t P, const float *psf,
                          int64_t pM, int64_t pN, int64_t pP, const dwhip_params *params,
                          float *out, dwhip_stats *stats, void *stream);
+
+/*
+ * dwhip_shb_run_device without any device allocation.  No counterpart in
+ * deconwolf: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The job's state lives in a static device block of
+ * DWHIP_SCRATCH_BYTES per GPU, held until the process ends; a job needing
+ * more (dwhip_job_bytes) returns DWHIP_ERR_SCRATCH before anything runs
+ * (256 MiB holds, for instance, a 128x128x32 image with a 127x127x63 PSF,
+ * transformed as 256x256x96).  Every call on a GPU shares that block:
+ * calls that overlap on one GPU race.  setup_ms and iter_ms read 0.
+ */
+#define DWHIP_SCRATCH_BYTES ((size_t)256 << 20)
+int dwhip_shb_run_device_noalloc(const float *im, int64_t M, int64_t N, int64_t P, const float *psf,
+                                 int64_t pM, int64_t pN, int64_t pP, const dwhip_params *params,
+                                 float *out, dwhip_stats *stats, void *stream);
 
 /*
  * Richardson-Lucy deconvolution, host buffers: the drop-in for

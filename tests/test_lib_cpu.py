@@ -98,7 +98,7 @@ def test_params_default_match_dw_opts_new():
 
 def test_strerror_and_job_bytes():
     L = _lib.lib()
-    for c in (0, -1, -2, -3, -4, -5, -6, -7):
+    for c in (0, -1, -2, -3, -4, -5, -6, -7, -8):
         assert len(L.dwhip_strerror(c)) > 0
     w = (ctypes.c_int64 * 3)(1250, 1215, 140)
     b = L.dwhip_job_bytes(w, 1024, 1024, 60, 181, 181, 79, 2, 50)
