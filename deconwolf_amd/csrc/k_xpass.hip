@@ -1,9 +1,25 @@
 // One X-pass mode (XMODE, set by the build) of the SHB loop.
-#include "shb_passes.h"
-
 #ifndef XMODE
 #error "compile with -DXMODE=<mode>"
 #endif
+
+// Per-mode build tunables of the two iteration passes: occupancy target
+// (waves per SIMD) and element chunk of the update (XM_UPDATE = 3) and
+// ratio (XM_RATIO = 4) passes, over the x-pass-wide DWHIP_X_WAVES / XCHUNK.
+#if XMODE == 3 && defined(DWHIP_XU_WAVES)
+#define DWHIP_X_WAVES DWHIP_XU_WAVES
+#endif
+#if XMODE == 4 && defined(DWHIP_XR_WAVES)
+#define DWHIP_X_WAVES DWHIP_XR_WAVES
+#endif
+#if XMODE == 3 && defined(DWHIP_XU_CHUNK)
+#define XCHUNK DWHIP_XU_CHUNK
+#endif
+#if XMODE == 4 && defined(DWHIP_XR_CHUNK)
+#define XCHUNK DWHIP_XR_CHUNK
+#endif
+
+#include "shb_passes.h"
 #define DWHIP_CAT2(a, b) a##b
 #define DWHIP_CAT(a, b) DWHIP_CAT2(a, b)
 

@@ -136,9 +136,15 @@ void save_locked()
     if (!f) return;
     for (const auto& kv : g_cache.plans) {
         const DwhipPlanChoice& c = kv.second;
+        // radices written out in full (an axis that kept the rule's plan
+        // has an empty list in memory)
+        const int len[3] = {(int)(kv.first[0] / 2), (int)kv.first[1], (int)kv.first[2]};
+        std::string r[3];
+        for (int a = 0; a < 3; ++a) r[a] = join(c.r[a].empty() ? dwhip_radices(len[a], a) : c.r[a]);
+        if (r[0].empty() || r[1].empty() || r[2].empty()) continue;  // length-1 axes: nothing to choose
         fprintf(f, "gfx950 %lld %lld %lld x=%s xrows=%d y=%s ycols=%d z=%s zcols=%d\n", (long long)kv.first[0],
-                (long long)kv.first[1], (long long)kv.first[2], join(c.r[0]).c_str(), c.xrows, join(c.r[1]).c_str(),
-                c.ycols, join(c.r[2]).c_str(), c.zcols);
+                (long long)kv.first[1], (long long)kv.first[2], r[0].c_str(), c.xrows, r[1].c_str(), c.ycols,
+                r[2].c_str(), c.zcols);
     }
     fclose(f);
     rename(tmp.c_str(), g_cache.path.c_str());
@@ -164,7 +170,8 @@ int time_axis(const Key& w, int method, const DwhipPlanChoice& c, int axis, void
     // the image and PSF are not read by the timed passes; the job sizes are
     // those of a job whose work size is w
     const float* none = reinterpret_cast<const float*>(mem);
-    int rc = dwhip_job_plan(&j, none, w[0] / 2, w[1] / 2, w[2] / 2, none, 1, 1, 1, &prm, nullptr);
+    int rc = dwhip_job_plan(&j, none, std::max<int64_t>(1, w[0] / 2), std::max<int64_t>(1, w[1] / 2),
+                            std::max<int64_t>(1, w[2] / 2), none, 1, 1, 1, &prm, nullptr);
     if (rc != DWHIP_OK) return rc;
     if (j.bytes > cap) return DWHIP_ERR_NOMEM;
     rc = dwhip_job_bind(&j, mem, s);
@@ -213,7 +220,8 @@ int tune_size(const Key& w, int method, DwhipPlanChoice* best, double* tuned, do
 
     // one block for every candidate (the layout depends on the plan only
     // through the twiddle table and the partial sums)
-    size_t cap = dwhip_job_bytes(wv, w[0] / 2, w[1] / 2, w[2] / 2, 1, 1, 1, 2, 2) + ((size_t)4 << 20);
+    size_t cap = dwhip_job_bytes(wv, std::max<int64_t>(1, w[0] / 2), std::max<int64_t>(1, w[1] / 2),
+                                 std::max<int64_t>(1, w[2] / 2), 1, 1, 1, 2, 2) + ((size_t)4 << 20);
     void* mem = nullptr;
     TCHK(hipMalloc(&mem, cap));
     hipStream_t s = nullptr;
@@ -284,13 +292,24 @@ bool dwhip_plan_cached(const int64_t* w, DwhipPlanChoice* pc)
 }
 
 // Tunes w unless a plan for it is cached (used by entry points whose
-// params ask for DWHIP_PLAN_MEASURE).
+// params ask for DWHIP_PLAN_MEASURE).  Measuring only chooses among plans
+// that all run: when it cannot run (no memory for its trial block beside
+// the caller's, a size with nothing to choose), the rule's plan is cached
+// for the size and the job goes on with it.
 int dwhip_plan_ensure(const int64_t* w, int method)
 {
+    (void)method;
     if (!dwhip_fast_size_ok(w)) return DWHIP_OK;  // the any-size path has one plan
     DwhipPlanChoice c;
     if (dwhip_plan_cached(w, &c)) return DWHIP_OK;
-    return dwhip_plan_tune(w, 0, nullptr, nullptr);
+    const int rc = dwhip_plan_tune(w, 0, nullptr, nullptr);
+    if (rc != DWHIP_OK) {
+        (void)hipGetLastError();  // a failed trial allocation leaves its error behind
+        std::lock_guard<std::mutex> lock(g_cache.mu);
+        load_locked();
+        g_cache.plans[Key{w[0], w[1], w[2]}] = DwhipPlanChoice();
+    }
+    return DWHIP_OK;
 }
 
 extern "C" int dwhip_plan_tune(const int64_t* work, int retune, double* tuned_ms, double* rule_ms)
