@@ -45,6 +45,9 @@ CPU_SAMPLE_ITERS = 50
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "shb_c2_pmc_traffic.md")
 METRIC = "SHB voxel-iter/s batched-served vs plain per-image; p50/p99 job latency"
+PLANNING = "measure"
+ROUNDS_PER_STEP = None
+PARITY_FILE = os.path.join(ROOT, "profiles", "parity_fullsize.md")
 
 
 def host_cores():
@@ -96,6 +99,25 @@ def run_cpu_baseline(npz, iters_full, threads, timeout_s):
     return d
 
 
+def parity_summary():
+    """{config: {oracle: rel-L2}} from the committed full-size parity summary
+    (the table of tests/test_gpu_parity_configs.py's figures), or None."""
+    out = {}
+    try:
+        for line in open(PARITY_FILE):
+            cols = [c.strip() for c in line.split("|")]
+            if len(cols) < 9 or not cols[2].isdigit():
+                continue
+            row = {"iterations": int(cols[2]), "vs_float32_rel_l2": float(cols[5]), "vs_float32_max_rel": float(cols[6])}
+            if cols[7] not in ("—", "-", ""):
+                row["vs_float64_rel_l2"] = float(cols[7])
+                row["vs_float64_max_rel"] = float(cols[8])
+            out[cols[1]] = row
+    except (OSError, ValueError):
+        return None
+    return out or None
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (scaled FETCH_SIZE + WRITE_SIZE), or None."""
@@ -109,12 +131,23 @@ def profiled_traffic(kernel):
     return None
 
 
+def job_opts(**kw):
+    """DwOpts of the benchmark's jobs: planning as --planning says."""
+    import deconwolf_amd as dw
+    s = dw.DwOpts(iter_type=dw.ITER_FIXED, **kw)
+    if PLANNING == "rule":
+        s.fftw3_planning = dw.method_shb.FFTW_ESTIMATE
+    return s
+
+
 def preplan(shapes):
     """Measures the transform plans of every distinct work size among
     (image shape, PSF shape) pairs before a timed region (dwhip_plan_tune,
     FFTW_MEASURE's analogue; cached, so the timed jobs find them); returns
-    the seconds it took."""
+    the seconds it took (0 with --planning rule)."""
     import deconwolf_amd as dw
+    if PLANNING == "rule":
+        return 0.0
     t = time.time()
     done = set()
     for ims, ps in shapes:
@@ -133,7 +166,7 @@ def _pct(xs, p):
     return xs[min(len(xs) - 1, int(round(p / 100.0 * (len(xs) - 1))))]
 
 
-def run_stream(n, rate, xy=None, fifo_leg=True):
+def run_stream(n, rate, xy=None, fifo_leg=True, reuse=False):
     """The c4 stream served by the batched engine and, for comparison, by
     the plain per-image path (FIFO, one job at a time), both against the
     same arrival times; inputs are generated and resident first.  xy: the
@@ -145,7 +178,7 @@ def run_stream(n, rate, xy=None, fifo_leg=True):
     from deconwolf_amd.serve import Engine, serve
 
     t = time.time()
-    jobs = workloads.make_stream(n, rate, xy=xy)
+    jobs = workloads.make_stream(n, rate, xy=xy, reuse=reuse)
     torch.cuda.synchronize()
     t_gen = time.time() - t
     t_plan = preplan([(tuple(j["im"].shape), tuple(j["psf"].shape)) for j in jobs])
@@ -153,7 +186,7 @@ def run_stream(n, rate, xy=None, fifo_leg=True):
     arrivals = [j["arrival"] for j in jobs]
 
     def opts(j):
-        return dw.DwOpts(nIter=j["iters"], iter_type=dw.ITER_FIXED)
+        return job_opts(nIter=j["iters"])
 
     # plain per-image, FIFO
     torch.cuda.synchronize()
@@ -174,7 +207,7 @@ def run_stream(n, rate, xy=None, fifo_leg=True):
 
     def run_engine(order, mvox):
         eng = Engine(hbm_budget_gb=budget, max_active=64, order=order, active_mvox=mvox, groups=ENGINE_GROUPS,
-                     aging_ms=SERVE_AGING_MS if order == _lib.ORDER_SHORTEST else 0)
+                     aging_ms=SERVE_AGING_MS if order == _lib.ORDER_SHORTEST else 0, rounds_per_step=ROUNDS_PER_STEP)
         reqs = [(i, j["im"], j["psf"], opts(j)) for i, j in enumerate(jobs)]
         times = serve(eng, reqs, arrivals)
         eng.close()
@@ -188,15 +221,16 @@ def run_stream(n, rate, xy=None, fifo_leg=True):
     del jobs
     torch.cuda.empty_cache()
     return {
-        "workload": workloads.describe_stream(n, rate, xy),
+        "workload": workloads.describe_stream(n, rate, xy) + (
+            "; one generated image per (XY, Z, wavelength) class, shared by its jobs" if reuse else ""),
         "voxel_iters": work,
         "served_value": work / served_s,
         "plain_value": work / plain_s,
         "served_vs_plain": plain_s / served_s,
         "served_p50_ms": 1e3 * _pct(lat_s, 50), "served_p99_ms": 1e3 * _pct(lat_s, 99),
         "plain_p50_ms": 1e3 * _pct(lat_p, 50), "plain_p99_ms": 1e3 * _pct(lat_p, 99),
-        "served_policy": "least work first (work / (1 + waited / %d ms)), at most %d M transform voxels iterating "
-                         "together" % (SERVE_AGING_MS, SERVE_MVOX),
+        "served_policy": "least work first, jobs waiting %d ms or more first (oldest first), at most %d M "
+                         "transform voxels iterating together" % (SERVE_AGING_MS, SERVE_MVOX),
         "served_fifo_all_value": work / fifo_s,
         "served_fifo_all_p50_ms": 1e3 * _pct(lat_f, 50), "served_fifo_all_p99_ms": 1e3 * _pct(lat_f, 99),
         "engine_hbm_budget_gb": budget,
@@ -216,6 +250,7 @@ def run_tiled():
 
     t = time.time()
     im, psf, s = workloads.make_c3()
+    s.fftw3_planning = job_opts().fftw3_planning
     torch.cuda.synchronize()
     t_gen = time.time() - t
     c = workloads.C3
@@ -228,7 +263,8 @@ def run_tiled():
     t_plan = preplan(shapes)
     res = {"workload": workloads.describe_c3(), "input_gen_s": t_gen, "planning_s": t_plan}
     for name, eng in (("plain_tiles", None), ("engine_tiles", "engine")):
-        e = Engine(hbm_budget_gb=64.0, max_active=64, groups=ENGINE_GROUPS) if eng else None
+        e = Engine(hbm_budget_gb=64.0, max_active=64, groups=ENGINE_GROUPS,
+                   rounds_per_step=ROUNDS_PER_STEP) if eng else None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out, tiles, _ = tiling.deconvolve_tiles(im, psf, s, c["tile"], c["overlap"], engine=e)
@@ -257,6 +293,8 @@ def run_c5(fields=1):
     jobs = []
     for f in range(fields):
         jobs += workloads.make_c5(f)
+    for j in jobs:
+        j["opts"].fftw3_planning = job_opts().fftw3_planning
     torch.cuda.synchronize()
     res = {"workload": workloads.describe_c5(fields), "input_gen_s": time.time() - t, "jobs": len(jobs)}
     res["planning_s"] = preplan([(tuple(j["im"].shape), tuple(j["psf"].shape)) for j in jobs])
@@ -270,7 +308,7 @@ def run_c5(fields=1):
     el = time.perf_counter() - t0
     res["plain_value"] = work / el
     res["plain_s"] = el
-    e = Engine(hbm_budget_gb=64.0, max_active=64, groups=ENGINE_GROUPS)
+    e = Engine(hbm_budget_gb=64.0, max_active=64, groups=ENGINE_GROUPS, rounds_per_step=ROUNDS_PER_STEP)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i, j in enumerate(jobs):
@@ -290,7 +328,7 @@ def run_c5(fields=1):
 
 
 def main():
-    global ENGINE_GROUPS
+    global ENGINE_GROUPS, SERVE_AGING_MS, PLANNING, ROUNDS_PER_STEP
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -312,11 +350,22 @@ def main():
                          "line with the table")
     ap.add_argument("--sweep-seconds", type=float, default=20.0)
     ap.add_argument("--sweep-xy", default="", help="XY sizes of the sweep's jobs (comma-separated; default all five)")
+    ap.add_argument("--rounds-per-step", type=int, default=None,
+                    help="rounds an engine step may run (default: the library's)")
+    ap.add_argument("--aging-ms", type=int, default=SERVE_AGING_MS,
+                    help="waiting bound of the served stream's least-work-first admission")
+    ap.add_argument("--planning", choices=("measure", "rule"), default="measure",
+                    help="transform planning: measure a new size's plans before the timed region (the library's "
+                         "default, FFTW_MEASURE's analogue) or plan by rule (FFTW_ESTIMATE; no trial launches, for "
+                         "profiles of the timed passes alone)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy a GIB-sized float32 buffer on the GPU 3 times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
     a = ap.parse_args()
     ENGINE_GROUPS = a.engine_groups
+    SERVE_AGING_MS = a.aging_ms
+    PLANNING = a.planning
+    ROUNDS_PER_STEP = a.rounds_per_step
     if a.copy_calibration > 0:
         import torch
         n = int(a.copy_calibration * (1 << 30)) // 4
@@ -335,7 +384,7 @@ def main():
         for r in (float(v) for v in a.sweep.split(",")):
             n = int(min(500, max(8, round(r * a.sweep_seconds))))
             t0 = time.time()
-            row = run_stream(n, r, xy=xy, fifo_leg=False)
+            row = run_stream(n, r, xy=xy, fifo_leg=False, reuse=True)
             row.update(rate=r, jobs=n, wall_s=time.time() - t0)
             print(json.dumps({"sweep_row": row}), flush=True)
             rows.append(row)
@@ -373,7 +422,7 @@ def main():
     torch.cuda.synchronize()
 
     def step():
-        st = dw.DwOpts(nIter=iters, iter_type=dw.ITER_FIXED)
+        st = job_opts(nIter=iters)
         dw.deconvolve_shb_hip_device(dim, dpsf, st, out=out)
         return st
 
@@ -456,6 +505,9 @@ def main():
             "all_passes_GBps": (all_by / 1e9) / (all_ms / 1e3),
         },
         "cpu_baseline": cpu,
+        "parity": parity_summary(),
+        "parity_source": "profiles/parity_fullsize.md (tests/test_gpu_parity_configs.py on MI355X, oracle at "
+                         "deconwolf's work size)",
         "served": served,
         "tiled": tiled,
         "c5": c5,

@@ -238,10 +238,12 @@ double job_work(const EngJob* j)
 
 // Admission: while the active set has room (max_active, the voxel cap and
 // the arena), the next job in the configured order that fits.  FIFO stops
-// at the first job that does not fit, so later jobs never overtake it;
-// shortest-first takes the least work among those that fit, the work of a
-// job that has waited t seconds counted as work / (1 + t / aging), so that
-// a large job is not passed over for ever (aging_ms 0: no aging).
+// at the first job that does not fit, so later jobs never overtake it.
+// Shortest-first takes the least work among those that fit, except that a
+// job that has waited aging_ms or more goes before every younger job
+// (oldest first) and, when it does not fit yet, keeps younger jobs from
+// taking the room the running jobs free (aging_ms 0: least work first
+// only).  The bound keeps a large job from being passed over for ever.
 int admit(dwhip_engine* E, std::vector<EngJob*>& fresh)
 {
     const double cap = (double)E->cfg.active_mvox * 1e6;
@@ -249,16 +251,18 @@ int admit(dwhip_engine* E, std::vector<EngJob*>& fresh)
     for (EngJob* j : E->active) vox += job_voxels(j);
     const double t = now_s();
     const double aging = E->cfg.aging_ms > 0 ? 1e-3 * E->cfg.aging_ms : 0;
-    auto score = [&](const EngJob* j) {
-        const double w = job_work(j);
-        return aging > 0 ? w / (1.0 + (t - j->submitted) / aging) : w;
-    };
+    auto aged = [&](const EngJob* j) { return aging > 0 && t - j->submitted >= aging; };
     while (!E->queue.empty() && (int)E->active.size() < E->cfg.max_active) {
         std::vector<size_t> order(E->queue.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = i;
         if (E->cfg.order == DWHIP_ORDER_SHORTEST)
-            std::stable_sort(order.begin(), order.end(),
-                             [&](size_t a, size_t b) { return score(E->queue[a]) < score(E->queue[b]); });
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+                const EngJob *ja = E->queue[a], *jb = E->queue[b];
+                const bool oa = aged(ja), ob = aged(jb);
+                if (oa != ob) return oa;                              // aged jobs first
+                if (oa) return ja->submitted < jb->submitted;         // oldest first among them
+                return job_work(ja) < job_work(jb);                   // then least work
+            });
         bool taken = false;
         for (size_t i : order) {
             EngJob* j = E->queue[i];
@@ -274,7 +278,7 @@ int admit(dwhip_engine* E, std::vector<EngJob*>& fresh)
                 taken = true;
                 break;
             }
-            if (E->cfg.order == DWHIP_ORDER_FIFO) break;
+            if (E->cfg.order == DWHIP_ORDER_FIFO || aged(j)) break;  // nobody overtakes it
         }
         if (!taken) break;
     }

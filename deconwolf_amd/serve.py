@@ -26,8 +26,8 @@ class Engine:
     groups on two streams whose passes overlap; rounds_per_step: rounds a
     step may run while every active job is FIXED (replayed from a captured
     HIP graph of the round), None for the library's default; aging_ms:
-    with ORDER_SHORTEST, a queued job's work counts as work / (1 + waited /
-    aging_ms)."""
+    with ORDER_SHORTEST, a job that has waited that long goes before every
+    younger one (oldest first), 0 for least work first only."""
 
     def __init__(self, hbm_budget_gb=32.0, max_active=64, stream=None, order=0, active_mvox=0, groups=1,
                  rounds_per_step=None, aging_ms=0):

@@ -128,14 +128,18 @@ def make_c4_job(M, P, lam, iters, rng, gen_seed, device="cuda:0", psf=None):
                 lam=lam)
 
 
-def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
+def make_stream(n, rate, seed=4, device="cuda:0", xy=None, reuse=False):
     """n jobs of the c4 stream: list of dicts with im, psf (torch tensors on
     device, prepared like dw_run), opts, arrival (s).  The PSFs come from
     the device Born-Wolf generator; blurring runs on the GPU through
-    torch.fft (input generation only)."""
+    torch.fft (input generation only).  reuse: one generated image per
+    (XY, Z, wavelength) class, shared by the jobs of that class (long
+    streams; the jobs' shapes, iteration counts and arrivals are drawn as
+    without it)."""
     rng = np.random.default_rng(seed)
     xy = xy or C4["xy"]
     psfs = {}
+    made = {}
     jobs = []
     t = 0.0
     for i in range(n):
@@ -146,7 +150,14 @@ def make_stream(n, rate, seed=4, device="cuda:0", xy=None):
         iters = int(rng.choice(C4["iters"]))
         if lam not in psfs:
             psfs[lam] = c4_psf(lam, device)
-        j = make_c4_job(M, P, lam, iters, rng, seed * 100003 + i, device=device, psf=psfs[lam])
+        key = (M, P, lam)
+        if reuse and key in made:
+            j = dict(made[key])
+            j["opts"] = DwOpts(nIter=iters, iter_type=2)
+            j["iters"] = iters
+        else:
+            j = make_c4_job(M, P, lam, iters, rng, seed * 100003 + i, device=device, psf=psfs[lam])
+            made[key] = j
         j["arrival"] = t
         jobs.append(j)
     return jobs

@@ -310,9 +310,10 @@ typedef struct {
                            while every active job is FIXED, a step runs rounds up to the
                            next completion, all but the last replayed from one captured
                            HIP graph of the round, with one host synchronisation */
-    int aging_ms;       /* DWHIP_ORDER_SHORTEST: a queued job's work counts as
-                           work / (1 + waited / aging_ms), so large jobs are not
-                           passed over for ever; 0 = no aging */
+    int aging_ms;       /* DWHIP_ORDER_SHORTEST: a job that has waited aging_ms or more
+                           goes before every younger job (oldest first), and younger
+                           jobs do not overtake it while it waits for room, so large
+                           jobs are not passed over for ever; 0 = least work first only */
     int reserved[1];
 } dwhip_engine_config;
 
