@@ -51,10 +51,20 @@ __device__ __forceinline__ double shb_alpha(int iter, float alphamax)
 }
 
 // p = x + alpha*(x - xp), then p = max(p, bg)  (method_shb.c:333-339):
-// x - xp in float, the rest in double, rounded to float.
+// x - xp in float, the rest in double, rounded to float.  With
+// DWHIP_EXTRAP_F32 the product and sum are one float FMA with alpha rounded
+// to float (one rounding of the result instead of the double's; a build
+// tunable to measure the double arithmetic's cost).
+#ifndef DWHIP_EXTRAP_F32
+#define DWHIP_EXTRAP_F32 0
+#endif
 __device__ __forceinline__ float shb_extrap(float x, float xp, double alpha, float bg)
 {
+#if DWHIP_EXTRAP_F32
+    float p = fmaf((float)alpha, x - xp, x);
+#else
     float p = (float)((double)x + alpha * (double)(x - xp));
+#endif
     return p < bg ? bg : p;
 }
 
@@ -244,6 +254,11 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
         // alpha of iterations k-1 and k (uniform)
         const double am1 = MODE == XM_UPDATE ? shb_alpha(iter - 1, J.alphamax) : 0.0;
         const double al = MODE == XM_UPDATE ? shb_alpha(iter, J.alphamax) : 0.0;
+        // the SHB ratio pass's tiles lie within one plane: its rows' y and z
+        // come from the tile (no division per element)
+        const bool plane_tile = MODE == XM_RATIO && !J.rl;
+        const int tz = plane_tile ? row0 / ny : 0;
+        const int ty0 = plane_tile ? row0 - tz * ny : 0;
         // chunks of XCHUNK elements per thread; full chunks carry no bounds
         // tests, so their loads and arithmetic schedule as straight-line code
         auto chunk = [&](int w0, auto full) {
@@ -257,8 +272,13 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                 const int bb = (int)fdiv((uint32_t)w, hmagic);
                 const int n = w - bb * h;
                 const int row = row0 + bb;
-                yy[q] = row % ny;
-                zz[q] = row / ny;
+                if (plane_tile) {
+                    yy[q] = ty0 + bb;
+                    zz[q] = tz;
+                } else {
+                    yy[q] = row % ny;
+                    zz[q] = row / ny;
+                }
                 m0[q] = 2 * n;
                 vi[q] = (size_t)row * nx + 2 * n;
                 a[q] = b[q] = c[q] = wv[q] = make_float2(0.f, 0.f);
