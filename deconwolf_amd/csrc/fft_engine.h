@@ -59,9 +59,12 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t j, uint32_t magic)
 {
     return magic ? __umulhi(j, magic) : j;
 }
+// ceil(2^32 / d) = floor((2^32 - 1) / d) + 1 for every d >= 2 (d divides
+// 2^32 only as a power of two, where both sides are 2^32 / d), so one 32-bit
+// division gives it; for d = 1 the sum wraps to the encoding 0.
 __host__ __device__ __forceinline__ uint32_t fdiv_magic(uint32_t d)
 {
-    return d <= 1 ? 0u : (uint32_t)((0x100000000ull + d - 1) / (uint64_t)d);
+    return d == 0 ? 0u : 0xffffffffu / d + 1u;
 }
 
 // ---------------------------------------------------------------------------

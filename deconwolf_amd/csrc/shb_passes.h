@@ -138,6 +138,15 @@ __host__ __device__ constexpr bool x_stage_rows(int mode) { return DWHIP_X_STAGE
 #endif
 __host__ __device__ constexpr bool x_nt(int mode) { return mode == XM_UPDATE ? DWHIP_NT_UPDATE : DWHIP_NT_X; }
 
+// Element loops of the two iteration passes specialised for their common
+// case (SHB iteration k > 0 of the update pass; the SHB ratio pass, whose
+// tiles hold image rows only): contiguous tile addressing, the job's flags
+// read once, selects instead of branches.  0 runs the generic loop for
+// every case (a build tunable for measurement).
+#ifndef DWHIP_X_FAST
+#define DWHIP_X_FAST 1
+#endif
+
 #ifdef DWHIP_X_WAVES
 #define DWHIP_X_OCC __attribute__((amdgpu_waves_per_eu(DWHIP_X_WAVES)))
 #else
@@ -412,9 +421,119 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                 lds[w] = out;
             }
                 };
-        int w0 = tid;
-        for (; w0 + (XCHUNK - 1) * XTHR < nwork; w0 += XCHUNK * XTHR) chunk(w0, BoolC<true>());
-        for (; w0 < nwork; w0 += XCHUNK * XTHR) chunk(w0, BoolC<false>());
+        // job flags read once, before the element loops' stores (the
+        // compiler otherwise reloads them from the job table per element)
+        const bool rl = J.rl;
+        const bool has_w = J.has_w;
+        const float lo = J.positivity ? bg : -INFINITY;  // x_k < lo -> lo (method_shb.c:366-374)
+        if (DWHIP_X_FAST && MODE == XM_UPDATE && !rl && iter > 0) {
+            // SHB iteration k > 0.  The tile's x-rows row0 .. row0+nrows-1
+            // are consecutive in every real volume (row stride nx = 2h), so
+            // packed pair w of the tile is pair w from the tile's first voxel.
+            const size_t tb = (size_t)row0 * nx;
+            const DWHIP_GL float2* __restrict__ xm1 = reinterpret_cast<const DWHIP_GL float2*>(Ukm1 + tb);
+            DWHIP_GL float2* __restrict__ xio = reinterpret_cast<DWHIP_GL float2*>(Uk + tb);  // x_{k-2} in, x_k out
+            const DWHIP_GL float2* __restrict__ wt = reinterpret_cast<const DWHIP_GL float2*>(W + tb);
+            auto upd = [&](int w0, auto full, auto hw) {
+                constexpr bool FULL = decltype(full)::value, HW = decltype(hw)::value;
+                float2 a[XCHUNK], b[XCHUNK], c[XCHUNK], wv[XCHUNK];
+#pragma unroll
+                for (int q = 0; q < XCHUNK; ++q) {
+                    const int w = w0 + q * XTHR;
+                    if (!FULL && w >= nwork) continue;
+                    a[q] = lds[w];
+                    b[q] = ldgs<XNT>(xm1 + w);
+                    c[q] = ldgs<XNT>(xio + w);
+                    wv[q] = HW ? ldgs<XNT>(wt + w) : make_float2(1.f, 1.f);
+                }
+#pragma unroll
+                for (int q = 0; q < XCHUNK; ++q) {
+                    const int w = w0 + q * XTHR;
+                    if (!FULL && w >= nwork) continue;
+                    // finish iteration k-1 (method_shb.c:55-70, 357-376)
+                    const float p0 = shb_extrap(b[q].x, c[q].x, am1, bg);
+                    const float p1 = shb_extrap(b[q].y, c[q].y, am1, bg);
+                    float2 xk = HW ? make_float2(a[q].x * (p0 * wv[q].x), a[q].y * (p1 * wv[q].y))
+                                   : make_float2(a[q].x * p0, a[q].y * p1);
+                    xk.x = xk.x < lo ? lo : xk.x;
+                    xk.y = xk.y < lo ? lo : xk.y;
+                    stgs<XNT>(xio + w, xk);
+                    lds[w] = make_float2(shb_extrap(xk.x, b[q].x, al, bg), shb_extrap(xk.y, b[q].y, al, bg));
+                }
+            };
+            auto rows = [&](auto hw) {
+                int w0 = tid;
+                for (; w0 + (XCHUNK - 1) * XTHR < nwork; w0 += XCHUNK * XTHR) upd(w0, BoolC<true>(), hw);
+                for (; w0 < nwork; w0 += XCHUNK * XTHR) upd(w0, BoolC<false>(), hw);
+            };
+            if (has_w)
+                rows(BoolC<true>());
+            else
+                rows(BoolC<false>());
+        } else if (DWHIP_X_FAST && MODE == XM_RATIO && plane_tile && h > 1) {
+            // SHB ratio over a plane tile: every row is an image row (y < N,
+            // z < P); pair w holds columns 2n, 2n+1 of tile row w / h.  Image
+            // loads are clamped into the tile's rows and dropped beyond
+            // column M (method_shb.c:27-48, getError dw.c:1436-1524).
+            const int M = J.M;
+            const DWHIP_GL float* __restrict__ imt = im + (size_t)M * (ty0 + (size_t)J.N * tz);
+            const int ilast = nrows * M - 1;
+            auto rat = [&](int w0, auto full, auto idiv) {
+                constexpr bool FULL = decltype(full)::value, IDIV = decltype(idiv)::value;
+                float2 a[XCHUNK], gv[XCHUNK];
+                int m0[XCHUNK];
+#pragma unroll
+                for (int q = 0; q < XCHUNK; ++q) {
+                    const int w = FULL ? w0 + q * XTHR : min(w0 + q * XTHR, nwork - 1);
+                    const int bb = (int)__umulhi((uint32_t)w, hmagic);  // h > 1: hmagic is the magic of h
+                    const int n = w - bb * h;
+                    m0[q] = 2 * n;
+                    const int i0 = bb * M + 2 * n;
+                    a[q] = lds[w];
+                    gv[q] = make_float2(imt[min(i0, ilast)], imt[min(i0 + 1, ilast)]);
+                }
+#pragma unroll
+                for (int q = 0; q < XCHUNK; ++q) {
+                    const int w = w0 + q * XTHR;
+                    if (!FULL && w >= nwork) continue;
+                    float r[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const bool valid = m0[q] + e < M;
+                        const float y = e ? a[q].y : a[q].x;
+                        const float g = e ? gv[q].y : gv[q].x;
+                        const double yd = y, gd = g;
+                        // the correctly rounded float quotient equals the
+                        // double quotient rounded to float, so get_fIdiv's
+                        // logf(gval / yval) needs no double division
+                        const float qt = g / y;
+                        double term;
+                        if (IDIV)
+                            term = (y > 0 && g > 0) ? gd * (double)logf(qt) - (gd - yd) : 0.0;
+                        else
+                            term = (yd - gd) * (yd - gd);
+                        acc += valid ? term : 0.0;
+                        float rv = qt;
+                        if (fabs(yd) < 1e-6) rv = g / (float)copysign(1e-6, yd);  // method_shb.c:41-42
+                        r[e] = valid ? rv : 0.f;
+                    }
+                    lds[w] = make_float2(r[0], r[1]);
+                }
+            };
+            auto rows = [&](auto idiv) {
+                int w0 = tid;
+                for (; w0 + (XCHUNK - 1) * XTHR < nwork; w0 += XCHUNK * XTHR) rat(w0, BoolC<true>(), idiv);
+                for (; w0 < nwork; w0 += XCHUNK * XTHR) rat(w0, BoolC<false>(), idiv);
+            };
+            if (J.metric == DWHIP_DEV_METRIC_IDIV)
+                rows(BoolC<true>());
+            else
+                rows(BoolC<false>());
+        } else {
+            int w0 = tid;
+            for (; w0 + (XCHUNK - 1) * XTHR < nwork; w0 += XCHUNK * XTHR) chunk(w0, BoolC<true>());
+            for (; w0 < nwork; w0 += XCHUNK * XTHR) chunk(w0, BoolC<false>());
+        }
         if (MODE == XM_RATIO) {
             const double s = block_sum<XTHR>(acc, red);
             if (tid == 0) gp(J.part)[lt] = s;
@@ -428,12 +547,10 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
             const uint32_t hpmagic = fdiv_magic((uint32_t)hp);
             DWHIP_GL float2* __restrict__ dst = gp(J.C);
             for (int w = tid; w < nrows * hp; w += XTHR) {
+                // (the SHB ratio pass's plane tiles hold image rows only, so
+                // every row of a tile is written)
                 const int b = (int)fdiv((uint32_t)w, hpmagic);
                 const int k = w - b * hp;
-                if (MODE == XM_RATIO && !J.rl) {
-                    const int row = row0 + b, zz = row / ny;
-                    if (zz >= J.P || row - zz * ny >= J.N) continue;
-                }
                 const float2* L = lds + b * h;
                 DWHIP_GL float2* crow = dst + (size_t)(row0 + b) * hcs;
                 if (k == 0) {
