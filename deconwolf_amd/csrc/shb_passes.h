@@ -111,6 +111,63 @@ __device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
 #define XCHUNK 8  // elements per thread whose loads are issued together
 #endif
 
+// C2R split source read straight from a tile's half-spectrum rows (row
+// stride hcs): element i of row pen is formed from X[i], X[h - i] and the
+// split twiddle.  With DWHIP_SPLIT_GROUP > 0 the loads of that many
+// elements of a radix-25 butterfly are issued before their splits are
+// formed; formed per element (0, the default), the compiler waits for each
+// element's loads before issuing the next element's.  Measured on the
+// update pass of a 1250-wide transform: 1084 us per element, 1208 us
+// grouped (182 VGPRs, two waves per SIMD), 1104 us grouped and held to
+// three waves; the waves of the other workgroups hide the per-element
+// latency at three waves per SIMD.
+#ifndef DWHIP_SPLIT_GROUP
+#define DWHIP_SPLIT_GROUP 0
+#endif
+template <bool NT>
+struct SplitRows {
+    static constexpr bool kStrided = true;
+    const DWHIP_GL float2* C;  // the tile's first row
+    TwPtr rtw;
+    int hcs, h;
+    __device__ __forceinline__ float2 operator()(int pen, int i) const  // one element (single-stage plans)
+    {
+        const DWHIP_GL float2* crow = C + (size_t)pen * hcs;
+        return c2r_split(ldgs<NT>(crow + i), ldgs<NT>(crow + h - i), ldg(rtw + i));
+    }
+    template <int R>
+    __device__ __forceinline__ void load(int pen, int j, int nb, float2* v) const
+    {
+        if constexpr (R != 25) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = (*this)(pen, j + r * nb);
+            return;
+        }
+        const DWHIP_GL float2* crow = C + (size_t)pen * hcs;
+        constexpr int G = DWHIP_SPLIT_GROUP > 0 && DWHIP_SPLIT_GROUP < R ? DWHIP_SPLIT_GROUP : R;
+#pragma unroll
+        for (int r0 = 0; r0 < R; r0 += G) {
+            float2 a[G], b[G], w[G];
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                if (r0 + r >= R) break;
+                const int i = j + (r0 + r) * nb;
+                a[r] = ldgs<NT>(crow + i);
+                b[r] = ldgs<NT>(crow + h - i);
+                w[r] = ldg(rtw + i);
+            }
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                if (r0 + r >= R) break;
+                v[r0 + r] = c2r_split(a[r], b[r], w[r]);
+            }
+            // the next group's loads stay behind this group's splits (its
+            // registers would otherwise be live across them)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+};
+
 template <bool B>
 struct BoolC {
     static constexpr bool value = B;
@@ -118,8 +175,16 @@ struct BoolC {
 
 // Passes that stage their half-spectrum rows in LDS before the C2R (faster
 // for the ratio pass, whose tiles are otherwise bound by the latency of the
-// split's scattered loads; slower for the update pass).
-__host__ __device__ constexpr bool x_stage_rows(int mode) { return DWHIP_X_STAGE_LDS && mode == XM_RATIO; }
+// split's scattered loads; slower for the update pass: DWHIP_X_STAGE_UPDATE
+// measured 1130 us against 1084 on a 1250-wide transform, 1157 us with the
+// nontemporal hint on its streams).
+#ifndef DWHIP_X_STAGE_UPDATE
+#define DWHIP_X_STAGE_UPDATE 0
+#endif
+__host__ __device__ constexpr bool x_stage_rows(int mode)
+{
+    return DWHIP_X_STAGE_LDS && (mode == XM_RATIO || (DWHIP_X_STAGE_UPDATE && mode == XM_UPDATE));
+}
 
 // Nontemporal hints on the passes' read-once/write-once streams.  Measured
 // on c2: the column passes and the ratio pass gain 2-10 %.  The update pass
@@ -153,20 +218,49 @@ __host__ __device__ constexpr bool x_nt(int mode) { return mode == XM_UPDATE ? D
 #define DWHIP_X_OCC
 #endif
 
-template <int MODE>
-__global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
+// Plans of the x-row transform compiled with their lengths known (h, the
+// stage geometry and the butterfly indexing fold to constants): two-stage
+// plans (R1, R2) of the forward transform, the inverse running R2, R1.
+// 25 x 25 is the x-row plan of 1250-wide transforms (1024-px images with
+// 181-px PSFs).  Off by default (a build tunable): with the geometry known
+// the compiler issues a whole butterfly's loads at once, and the passes
+// need 218-238 VGPRs (two waves per SIMD; measured 6 % slower on the update
+// pass, 2 % slower still when held to three waves with spills).
+#ifndef DWHIP_X_FIXED
+#define DWHIP_X_FIXED 0
+#endif
+__host__ __device__ constexpr bool x_fixed_mode(int mode)
 {
-    extern __shared__ float2 lds[];
-    __shared__ double red[XTHR / 64];
+    return DWHIP_X_FIXED && (mode == XM_UPDATE || mode == XM_RATIO);
+}
+
+// x-row transform of a tile with plan p, or with the compile-time plan
+// (FR1, FR2) when FR1 > 0 (p then gives only the stage twiddle offsets)
+template <int FR1, int FR2, int DIR, bool SRC_LDS, bool DST_LDS, class Src, class Dst>
+__device__ __forceinline__ void x_fft(const FftPlanDev& p, const StageGeo& g, int tid, float2* lds, TwPtr __restrict__ tw,
+                                      Src src, Dst dst)
+{
+    if constexpr (FR1 > 0) {
+        constexpr int RA = DIR < 0 ? FR1 : FR2, RB = DIR < 0 ? FR2 : FR1;
+        stage_first<RA, mb_for(DWHIP_XMAXE, RA), DIR, false, SRC_LDS>(g, tid, src, lds);
+        __syncthreads();
+        stage_last<RB, mb_for(DWHIP_XMAXE, RB), DIR, false, DST_LDS>(g, tid, lds, tw + p.twoff[1], dst);
+    } else {
+        fft_tile<DWHIP_XMAXE, DIR, false, DWHIP_RMAX_X, SRC_LDS, DST_LDS>(p, g, tid, lds, tw, src, dst);
+    }
+}
+
+// One X tile t of job J; FR1 > 0: the job's x plan is (FR1, FR2).
+template <int MODE, int FR1, int FR2>
+__device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, float2* lds, double* red)
+{
     constexpr bool XNT = x_nt(MODE);
-    const int tid = threadIdx.x;
     constexpr int kind = xmode_kind(MODE);
-    const int total = bd->ntile[kind];
-    const int njobs = bd->njobs;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
+    {
         const int lt = t - J.tile0[kind];
-        const int h = J.h, nx = J.nx, ny = J.ny, hcs = J.hcs;
+        const int h = FR1 > 0 ? FR1 * FR2 : J.h;
+        const int nx = FR1 > 0 ? 2 * h : J.nx;
+        const int ny = J.ny, hcs = J.hcs;
         int row0 = lt * J.xrows;
         int nrows = min(J.xrows, ny * J.nz - row0);
         if (MODE == XM_RATIO && !J.rl) {
@@ -234,7 +328,11 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                 const float2* L = lds + pen * hc;
                 return c2r_split(L[i], L[h - i], ldg(rtw + i));
             };
-            fft_tile<DWHIP_XMAXE, +1, false, DWHIP_RMAX_X, true, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
+            x_fft<FR1, FR2, +1, true, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
+            __syncthreads();
+        } else if (inv && DWHIP_SPLIT_GROUP > 0 && MODE != XM_RATIO) {
+            SplitRows<XNT> split_src{C + (size_t)row0 * hcs, rtw, hcs, h};
+            x_fft<FR1, FR2, +1, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
             __syncthreads();
         } else if (inv) {
             auto split_src = [&](int pen, int i) {
@@ -246,7 +344,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
                 const DWHIP_GL float2* crow = C + (size_t)row * hcs;
                 return c2r_split(ldgs<XNT>(crow + i), ldgs<XNT>(crow + h - i), ldg(rtw + i));
             };
-            fft_tile<DWHIP_XMAXE, +1, false, DWHIP_RMAX_X, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
+            x_fft<FR1, FR2, +1, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
             __syncthreads();
         }
 
@@ -541,7 +639,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
         __syncthreads();
 
         if (fwd) {
-            fft_tile<DWHIP_XMAXE, -1, false, DWHIP_RMAX_X, true, true>(J.pxf, g, tid, lds, gp(J.tw), lds_src, lds_dst);
+            x_fft<FR1, FR2, -1, true, true>(J.pxf, g, tid, lds, gp(J.tw), lds_src, lds_dst);
             __syncthreads();
             const int hp = h / 2 + 1;
             const uint32_t hpmagic = fdiv_magic((uint32_t)hp);
@@ -565,6 +663,26 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
             }
         }
         __syncthreads();
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
+{
+    extern __shared__ float2 lds[];
+    __shared__ double red[XTHR / 64];
+    constexpr int kind = xmode_kind(MODE);
+    const int total = bd->ntile[kind];
+    const int njobs = bd->njobs;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
+        if constexpr (x_fixed_mode(MODE)) {
+            if (J.pxf.nst == 2 && J.pxf.radix[0] == 25 && J.pxf.radix[1] == 25) {
+                xpass_tile<MODE, 25, 25>(J, t, threadIdx.x, lds, red);
+                continue;
+            }
+        }
+        xpass_tile<MODE, 0, 0>(J, t, threadIdx.x, lds, red);
     }
 }
 
