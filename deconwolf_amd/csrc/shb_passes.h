@@ -121,6 +121,12 @@ __device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
 // grouped (182 VGPRs, two waves per SIMD), 1104 us grouped and held to
 // three waves; the waves of the other workgroups hide the per-element
 // latency at three waves per SIMD.
+// R2C merge of a tile: by bin (a thread owns bin k of every row, one pair
+// of twiddle loads) or, with 0, by element (a twiddle load per element).
+#ifndef DWHIP_MERGE_BY_BIN
+#define DWHIP_MERGE_BY_BIN 1
+#endif
+
 #ifndef DWHIP_SPLIT_GROUP
 #define DWHIP_SPLIT_GROUP 0
 #endif
@@ -642,15 +648,36 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
             x_fft<FR1, FR2, -1, true, true>(J.pxf, g, tid, lds, gp(J.tw), lds_src, lds_dst);
             __syncthreads();
             const int hp = h / 2 + 1;
+            DWHIP_GL float2* __restrict__ dst = gp(J.C) + (size_t)row0 * hcs;
+            // (the SHB ratio pass's plane tiles hold image rows only, so
+            // every row of a tile is written)
+#if DWHIP_MERGE_BY_BIN
+            // a thread merges bin k and its mirror h - k of every row of the
+            // tile, with one pair of twiddles
+            for (int k = tid; k < hp; k += XTHR) {
+                const float2 wk = ldg(rtw + k);
+                const float2 wm = ldg(rtw + (k == 0 ? 0 : h - k));
+                for (int b = 0; b < nrows; ++b) {
+                    const float2* L = lds + b * h;
+                    DWHIP_GL float2* crow = dst + (size_t)b * hcs;
+                    if (k == 0) {
+                        const float2 z0 = L[0];
+                        stgs<XNT>(crow, make_float2(z0.x + z0.y, 0.f));
+                        stgs<XNT>(crow + h, make_float2(z0.x - z0.y, 0.f));
+                    } else {
+                        const float2 a = L[k], c = L[h - k];
+                        stgs<XNT>(crow + k, r2c_merge(a, c, wk));
+                        if (2 * k != h) stgs<XNT>(crow + h - k, r2c_merge(c, a, wm));
+                    }
+                }
+            }
+#else
             const uint32_t hpmagic = fdiv_magic((uint32_t)hp);
-            DWHIP_GL float2* __restrict__ dst = gp(J.C);
             for (int w = tid; w < nrows * hp; w += XTHR) {
-                // (the SHB ratio pass's plane tiles hold image rows only, so
-                // every row of a tile is written)
                 const int b = (int)fdiv((uint32_t)w, hpmagic);
                 const int k = w - b * hp;
                 const float2* L = lds + b * h;
-                DWHIP_GL float2* crow = dst + (size_t)(row0 + b) * hcs;
+                DWHIP_GL float2* crow = dst + (size_t)b * hcs;
                 if (k == 0) {
                     const float2 z0 = L[0];
                     stgs<XNT>(crow, make_float2(z0.x + z0.y, 0.f));
@@ -661,6 +688,7 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
                     if (2 * k != h) stgs<XNT>(crow + h - k, r2c_merge(c, a, ldg(rtw + h - k)));
                 }
             }
+#endif
         }
         __syncthreads();
     }
