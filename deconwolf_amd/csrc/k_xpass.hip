@@ -6,6 +6,11 @@
 // Per-mode build tunables of the two iteration passes: occupancy target
 // (waves per SIMD) and element chunk of the update (XM_UPDATE = 3) and
 // ratio (XM_RATIO = 4) passes, over the x-pass-wide DWHIP_X_WAVES / XCHUNK.
+// The ratio pass is held to three waves per SIMD (168 VGPRs; its LDS tiles
+// allow three workgroups per CU): without the bound the compiler takes 170.
+#if XMODE == 4 && !defined(DWHIP_XR_WAVES) && !defined(DWHIP_X_WAVES)
+#define DWHIP_XR_WAVES 3
+#endif
 #if XMODE == 3 && defined(DWHIP_XU_WAVES)
 #define DWHIP_X_WAVES DWHIP_XU_WAVES
 #endif

@@ -130,6 +130,48 @@ __device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
 #ifndef DWHIP_SPLIT_GROUP
 #define DWHIP_SPLIT_GROUP 0
 #endif
+// Split twiddles of a first-stage butterfly: element i = j + r*h/R needs
+// exp(-2 pi i i/nx) = exp(-2 pi i j/nx) * exp(-i pi r/R), one table load
+// and R - 1 products by compile-time constants instead of R table loads
+// (build tunables).  Measured on a 1250-wide transform: the ratio pass's
+// LDS-staged split 316 -> 313 us; the update pass's split from global
+// memory 1032 -> 1149 us (182 VGPRs), so that one stays off.
+#ifndef DWHIP_SPLIT_ROT_STAGED
+#define DWHIP_SPLIT_ROT_STAGED 1
+#endif
+#ifndef DWHIP_SPLIT_ROT_ROWS
+#define DWHIP_SPLIT_ROT_ROWS 0
+#endif
+template <int R>
+__device__ __forceinline__ float2 split_tw(float2 w0, int r)
+{
+    return r == 0 ? w0 : cmul(w0, make_float2(Roots<2 * R>::t.c[r], -Roots<2 * R>::t.s[r]));
+}
+
+// C2R split source from half-spectrum rows staged in LDS (row stride hc)
+struct SplitStaged {
+    static constexpr bool kStrided = true;
+    const float2* L;
+    TwPtr rtw;
+    int hc, h;
+    __device__ __forceinline__ float2 operator()(int pen, int i) const
+    {
+        const float2* Lr = L + pen * hc;
+        return c2r_split(Lr[i], Lr[h - i], ldg(rtw + i));
+    }
+    template <int R>
+    __device__ __forceinline__ void load(int pen, int j, int nb, float2* v) const
+    {
+        const float2* Lr = L + pen * hc;
+        const float2 w0 = ldg(rtw + j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = j + r * nb;
+            v[r] = c2r_split(Lr[i], Lr[h - i], split_tw<R>(w0, r));
+        }
+    }
+};
+
 template <bool NT>
 struct SplitRows {
     static constexpr bool kStrided = true;
@@ -144,7 +186,17 @@ struct SplitRows {
     template <int R>
     __device__ __forceinline__ void load(int pen, int j, int nb, float2* v) const
     {
-        if constexpr (R != 25) {
+        if constexpr (DWHIP_SPLIT_ROT_ROWS) {
+            const DWHIP_GL float2* crow = C + (size_t)pen * hcs;
+            const float2 w0 = ldg(rtw + j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = j + r * nb;
+                v[r] = c2r_split(ldgs<NT>(crow + i), ldgs<NT>(crow + h - i), split_tw<R>(w0, r));
+            }
+            return;
+        }
+        if constexpr (R != 25 || DWHIP_SPLIT_GROUP == 0) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[r] = (*this)(pen, j + r * nb);
             return;
@@ -330,13 +382,18 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
                     if (w0 + q * XTHR < nraw) lds[w0 + q * XTHR] = v[q];
             }
             __syncthreads();
-            auto split_src = [&](int pen, int i) {
-                const float2* L = lds + pen * hc;
-                return c2r_split(L[i], L[h - i], ldg(rtw + i));
-            };
-            x_fft<FR1, FR2, +1, true, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
+            if (DWHIP_SPLIT_ROT_STAGED) {
+                SplitStaged split_src{lds, rtw, hc, h};
+                x_fft<FR1, FR2, +1, true, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
+            } else {
+                auto split_src = [&](int pen, int i) {
+                    const float2* L = lds + pen * hc;
+                    return c2r_split(L[i], L[h - i], ldg(rtw + i));
+                };
+                x_fft<FR1, FR2, +1, true, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
+            }
             __syncthreads();
-        } else if (inv && DWHIP_SPLIT_GROUP > 0 && MODE != XM_RATIO) {
+        } else if (inv && (DWHIP_SPLIT_GROUP > 0 || DWHIP_SPLIT_ROT_ROWS) && MODE != XM_RATIO) {
             SplitRows<XNT> split_src{C + (size_t)row0 * hcs, rtw, hcs, h};
             x_fft<FR1, FR2, +1, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
             __syncthreads();
