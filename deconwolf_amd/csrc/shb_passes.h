@@ -121,10 +121,12 @@ __device__ __forceinline__ float2 r2c_merge(float2 a, float2 b, float2 w)
 // grouped (182 VGPRs, two waves per SIMD), 1104 us grouped and held to
 // three waves; the waves of the other workgroups hide the per-element
 // latency at three waves per SIMD.
-// R2C merge of a tile: by bin (a thread owns bin k of every row, one pair
-// of twiddle loads) or, with 0, by element (a twiddle load per element).
+// R2C merge of a tile: 2 by output bin (a thread writes bin m of every
+// row, one twiddle load), 1 by bin pair (bins k and h - k of every row, two
+// twiddle loads; a wave's mirrored stores leave partly written lines), 0
+// by element (a twiddle load per element).
 #ifndef DWHIP_MERGE_BY_BIN
-#define DWHIP_MERGE_BY_BIN 1
+#define DWHIP_MERGE_BY_BIN 2
 #endif
 
 #ifndef DWHIP_SPLIT_GROUP
@@ -394,7 +396,7 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
             }
             __syncthreads();
         } else if (inv && (DWHIP_SPLIT_GROUP > 0 || DWHIP_SPLIT_ROT_ROWS) && MODE != XM_RATIO) {
-            SplitRows<XNT> split_src{C + (size_t)row0 * hcs, rtw, hcs, h};
+            SplitRows<false> split_src{C + (size_t)row0 * hcs, rtw, hcs, h};
             x_fft<FR1, FR2, +1, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
             __syncthreads();
         } else if (inv) {
@@ -404,8 +406,12 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
                     const int zz = row / ny;
                     if (zz >= J.P || row - zz * ny >= J.N) return make_float2(0.f, 0.f);
                 }
+                // plain loads: each spectrum line is read twice (elements k
+                // and h - k), and a nontemporal hint on the first read drops
+                // the line before the second (the weights pass fetched its
+                // spectrum twice)
                 const DWHIP_GL float2* crow = C + (size_t)row * hcs;
-                return c2r_split(ldgs<XNT>(crow + i), ldgs<XNT>(crow + h - i), ldg(rtw + i));
+                return c2r_split(ldg(crow + i), ldg(crow + h - i), ldg(rtw + i));
             };
             x_fft<FR1, FR2, +1, false, true>(J.pxi, g, tid, lds, gp(J.tw), split_src, lds_dst);
             __syncthreads();
@@ -704,11 +710,29 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
         if (fwd) {
             x_fft<FR1, FR2, -1, true, true>(J.pxf, g, tid, lds, gp(J.tw), lds_src, lds_dst);
             __syncthreads();
-            const int hp = h / 2 + 1;
+            [[maybe_unused]] const int hp = h / 2 + 1;
             DWHIP_GL float2* __restrict__ dst = gp(J.C) + (size_t)row0 * hcs;
             // (the SHB ratio pass's plane tiles hold image rows only, so
             // every row of a tile is written)
-#if DWHIP_MERGE_BY_BIN
+#if DWHIP_MERGE_BY_BIN == 2
+            // a thread writes bin m (0 <= m <= h) of every row of the tile,
+            // X[m] = merge(Z[m], Z[h - m], exp(-2 pi i m/nx)), with one twiddle;
+            // a wave's stores cover whole lines of each row
+            for (int m = tid; m <= h; m += XTHR) {
+                const float2 wm = ldg(rtw + (m < h ? m : 0));
+                for (int b = 0; b < nrows; ++b) {
+                    const float2* L = lds + b * h;
+                    float2 v;
+                    if (m == 0 || m == h) {
+                        const float2 z0 = L[0];
+                        v = make_float2(m == 0 ? z0.x + z0.y : z0.x - z0.y, 0.f);
+                    } else {
+                        v = r2c_merge(L[m], L[h - m], wm);
+                    }
+                    stgs<XNT>(dst + (size_t)b * hcs + m, v);
+                }
+            }
+#elif DWHIP_MERGE_BY_BIN
             // a thread merges bin k and its mirror h - k of every row of the
             // tile, with one pair of twiddles
             for (int k = tid; k < hp; k += XTHR) {
