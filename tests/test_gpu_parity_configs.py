@@ -18,7 +18,8 @@ oracle's own transforms on a small case.
   39 planes by psf_autocrop) and 1024x1024x80, in one engine, 20
   iterations (the smallest count of the stream), float32.
 * c5: one channel (690 nm, 1024x1024x81, PSF 181x181x81 at dz 300 nm), all
-  100 iterations, float32 and float64.
+  100 iterations, float32 and float64; the other channels of the field on
+  request (DWHIP_PARITY_C5_CHANNELS).
 
 Tolerance: relative L2 <= 1e-4 (the bar deconwolf's float SHB output is
 held to) for the short runs; for the 50- and 100-iteration runs the bar
@@ -170,14 +171,22 @@ def test_c2_border_quality_1_exact_size(mkl):
     np.testing.assert_allclose(s.err_trace, e, rtol=1e-4)
 
 
-def test_c5_channel_all_iterations(mkl):
-    print("c5: one channel (690 nm), 100 iterations, float32 and float64 oracles", flush=True)
-    job = workloads.make_c5(0, channels=(3,), device=DEV)[0]
+# channels of field 0 checked (each about 3 minutes of oracle time on 16
+# host threads): the 690 nm channel by default, any of 0-3 (460, 520, 600,
+# 690 nm) through DWHIP_PARITY_C5_CHANNELS, e.g. "0,1,2,3"
+C5_CHANNELS = [int(c) for c in os.environ.get("DWHIP_PARITY_C5_CHANNELS", "3").split(",") if c.strip()]
+
+
+@pytest.mark.parametrize("ch", C5_CHANNELS)
+def test_c5_channel_all_iterations(mkl, ch):
+    lam = workloads.C5["lambdas"][ch]
+    print("c5: channel %d (%g nm), 100 iterations, float32 and float64 oracles" % (ch, lam), flush=True)
+    job = workloads.make_c5(0, channels=(ch,), device=DEV)[0]
     assert job["opts"].nIter == 100
     im = job["im"].cpu().numpy()
     psf = job["psf"].cpu().numpy()
     del job
-    _against_oracles("c5 channel 690 nm", im, psf, 100, mkl)
+    _against_oracles("c5 channel %g nm" % lam, im, psf, 100, mkl)
 
 
 def test_c4_extreme_jobs_in_engine(mkl):
