@@ -47,6 +47,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "shb_c2_pmc_traffic.md")
 METRIC = "SHB voxel-iter/s batched-served vs plain per-image; p50/p99 job latency"
 PLANNING = "measure"
 ROUNDS_PER_STEP = None
+STREAM_REL = None  # err_rel of the request stream's REL stopping (None: FIXED jobs)
 PARITY_FILE = os.path.join(ROOT, "profiles", "parity_fullsize.md")
 
 
@@ -132,9 +133,11 @@ def profiled_traffic(kernel):
 
 
 def job_opts(**kw):
-    """DwOpts of the benchmark's jobs: planning as --planning says."""
+    """DwOpts of the benchmark's jobs (FIXED unless kw says otherwise):
+    planning as --planning says."""
     import deconwolf_amd as dw
-    s = dw.DwOpts(iter_type=dw.ITER_FIXED, **kw)
+    kw.setdefault("iter_type", dw.ITER_FIXED)
+    s = dw.DwOpts(**kw)
     if PLANNING == "rule":
         s.fftw3_planning = dw.method_shb.FFTW_ESTIMATE
     return s
@@ -186,21 +189,31 @@ def run_stream(n, rate, xy=None, fifo_leg=True, reuse=False):
     arrivals = [j["arrival"] for j in jobs]
 
     def opts(j):
+        if STREAM_REL:
+            # REL stopping (dw_iterator_next) with the drawn count as maxiter
+            return job_opts(nIter=j["iters"], iter_type=dw.ITER_REL, err_rel=STREAM_REL, maxiter=j["iters"])
         return job_opts(nIter=j["iters"])
 
     # plain per-image, FIFO
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lat_p, last = [], 0.0
+    ran = []  # iterations each job ran (the drawn count unless REL stops it)
     for j in jobs:
         now = time.perf_counter() - t0
         if j["arrival"] > now:
             time.sleep(j["arrival"] - now)
-        dw.deconvolve_shb_hip_device(j["im"], j["psf"], opts(j))
+        s = opts(j)
+        dw.deconvolve_shb_hip_device(j["im"], j["psf"], s)
         torch.cuda.synchronize()
         last = time.perf_counter() - t0
         lat_p.append(last - j["arrival"])
+        ran.append(s.stats["iterations"])
     plain_s = last - arrivals[0]
+    if STREAM_REL:
+        # the engine's jobs stop after the same iterations (the same rule on
+        # the same errors); the work is what ran
+        work = sum(j["shape"][0] * j["shape"][1] * j["shape"][2] * r for j, r in zip(jobs, ran))
 
     free, total = torch.cuda.mem_get_info()
     budget = max(4.0, 0.8 * free / (1 << 30))
@@ -222,7 +235,9 @@ def run_stream(n, rate, xy=None, fifo_leg=True, reuse=False):
     torch.cuda.empty_cache()
     return {
         "workload": workloads.describe_stream(n, rate, xy) + (
-            "; one generated image per (XY, Z, wavelength) class, shared by its jobs" if reuse else ""),
+            "; one generated image per (XY, Z, wavelength) class, shared by its jobs" if reuse else "") + (
+            "; REL stopping, err_rel %g, the drawn iteration count as maxiter (%d iterations run in all)" %
+            (STREAM_REL, sum(ran)) if STREAM_REL else ""),
         "voxel_iters": work,
         "served_value": work / served_s,
         "plain_value": work / plain_s,
@@ -339,7 +354,7 @@ def run_c5(fields=1, chunk=4):
 
 
 def main():
-    global ENGINE_GROUPS, SERVE_AGING_MS, PLANNING, ROUNDS_PER_STEP
+    global ENGINE_GROUPS, SERVE_AGING_MS, PLANNING, ROUNDS_PER_STEP, STREAM_REL
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -361,6 +376,9 @@ def main():
                          "line with the table")
     ap.add_argument("--sweep-seconds", type=float, default=20.0)
     ap.add_argument("--sweep-xy", default="", help="XY sizes of the sweep's jobs (comma-separated; default all five)")
+    ap.add_argument("--stream-rel", type=float, default=None, metavar="ERR_REL",
+                    help="request-stream jobs stop by dw_iterator_next's REL rule (err_rel ERR_REL, the drawn "
+                         "iteration count as maxiter) instead of running a fixed count")
     ap.add_argument("--rounds-per-step", type=int, default=None,
                     help="rounds an engine step may run (default: the library's)")
     ap.add_argument("--aging-ms", type=int, default=SERVE_AGING_MS,
@@ -377,6 +395,7 @@ def main():
     SERVE_AGING_MS = a.aging_ms
     PLANNING = a.planning
     ROUNDS_PER_STEP = a.rounds_per_step
+    STREAM_REL = a.stream_rel
     if a.copy_calibration > 0:
         import torch
         n = int(a.copy_calibration * (1 << 30)) // 4

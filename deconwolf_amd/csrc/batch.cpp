@@ -164,6 +164,14 @@ void table_free(Table* t)
 }
 
 // Fills t with jobs and queues its upload on s.
+// REL/ABS jobs of the fused passes stop on the device: the end-of-round
+// kernel applies dw_iterator_next and the passes skip a stopped job, so
+// such jobs join multi-round steps; any-size jobs decide on the host
+bool dev_stop_job(const EngJob* j)
+{
+    return j->prm.iter_type != DWHIP_ITER_FIXED && !j->j.generic;
+}
+
 int table_build(Table* t, const std::vector<EngJob*>& jobs, hipStream_t s)
 {
     if ((int)jobs.size() > t->cap) return DWHIP_ERR_ARG;
@@ -179,6 +187,7 @@ int table_build(Table* t, const std::vector<EngJob*>& jobs, hipStream_t s)
     }
     for (int i = 0; i < t->n; ++i) {
         jh[i] = jobs[i]->j.d;
+        jh[i].dev_stop = dev_stop_job(jobs[i]) ? 1 : 0;
         for (int k = 0; k < KIND_N; ++k) {
             jh[i].tile0[k] = t->ntile[k];
             t->ntile[k] += jh[i].ntile[k];
@@ -219,7 +228,8 @@ bool keep_going(const EngJob* j, int iter)
     if (p->iter_type == DWHIP_ITER_REL) {
         float last = j->prev_err;
         if (iter < 2) last = 2 * j->last_err * p->err_rel;
-        if (fabs(j->last_err - last) / j->last_err < p->err_rel) return false;
+        // C's fabs: the float difference, then double arithmetic
+        if (fabs((double)(j->last_err - last)) / (double)j->last_err < (double)p->err_rel) return false;
     } else if (p->iter_type == DWHIP_ITER_ABS) {
         if (iter > 0 && j->last_err < p->err_abs) return false;
     }
@@ -568,18 +578,19 @@ static int capture_round(void* ctx, hipStream_t cs)
     return DWHIP_OK;
 }
 
-// Rounds this step runs: with every active job in FIXED mode, up to the
-// next completion (the fewest rounds any job has left), at most
-// rounds_per_step; one round otherwise (REL/ABS decide after each round),
-// and while kernels are profiled (events per pass).
+// Rounds this step runs: up to the next known completion (the fewest
+// rounds any job has left before nIter, or maxiter for REL/ABS jobs, which
+// may stop earlier on the device), at most rounds_per_step; one round with
+// an any-size job active (its REL/ABS rule runs on the host after each
+// round) and while kernels are profiled (events per pass).
 static int step_rounds(const dwhip_engine* E)
 {
     const int cap = std::max(1, E->cfg.rounds_per_step);
     if (cap == 1 || dwhip_prof_on() || dwhip_graph_mode() == 0) return 1;
     int r = cap;
     for (const EngJob* j : E->active) {
-        if (j->prm.iter_type != DWHIP_ITER_FIXED || j->j.generic) return 1;
-        r = std::min(r, j->prm.niter - j->j.host_iter);
+        if (j->j.generic) return 1;
+        r = std::min(r, iter_target(j) - j->j.host_iter);
     }
     return std::max(1, r);
 }
@@ -664,12 +675,17 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         j->nrun = j->j.host_iter;
     }
     E->rounds += 1;
-    // errors of this round, for REL/ABS jobs
+    // REL/ABS jobs: the rounds run and the stop flag (fused passes), or the
+    // error of this round (any-size jobs, whose rule runs here)
     std::vector<float> errs(E->active.size(), 0.f);
+    std::vector<int> iter_stop(2 * E->active.size(), 0);
     bool need_err = false;
     for (size_t i = 0; i < E->active.size(); ++i) {
         EngJob* j = E->active[i];
-        if (j->prm.iter_type != DWHIP_ITER_FIXED) {
+        if (dev_stop_job(j)) {
+            ECHK(hipMemcpyAsync(&iter_stop[2 * i], &j->j.d.st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, js[j]));
+            need_err = true;
+        } else if (j->prm.iter_type != DWHIP_ITER_FIXED) {
             ECHK(hipMemcpyAsync(&errs[i], j->j.d.err + (j->nrun - 1), sizeof(float), hipMemcpyDeviceToHost, js[j]));
             need_err = true;
         }
@@ -685,6 +701,9 @@ int dwhip_engine_step(dwhip_engine* E, int64_t* done_tags, int max_done, int* n_
         bool fin;
         if (j->prm.iter_type == DWHIP_ITER_FIXED) {
             fin = j->nrun >= j->prm.niter;
+        } else if (dev_stop_job(j)) {
+            j->j.host_iter = j->nrun = iter_stop[2 * i];
+            fin = iter_stop[2 * i + 1] != 0;
         } else {
             j->prev_err = j->last_err;
             j->last_err = errs[i];

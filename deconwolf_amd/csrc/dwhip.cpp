@@ -508,6 +508,10 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.start_identity = prm->start != DWHIP_START_FLAT;
     d.niter = niter;
     d.rl = prm->method == DWHIP_METHOD_RL;
+    d.iter_type = prm->iter_type;
+    d.err_rel = prm->err_rel;
+    d.err_abs = prm->err_abs;
+    d.dev_stop = 0;
     d.alphamax = prm->alphamax;
     d.bg_user = prm->bg;
     d.im = im_dev;
@@ -761,7 +765,8 @@ static bool iterator_continue(const dwhip_params* p, int iter, float error, floa
 {
     if (p->iter_type == DWHIP_ITER_REL) {
         if (iter < 2) lasterror = 2 * error * p->err_rel;
-        if (fabs(error - lasterror) / error < p->err_rel) return false;
+        // C's fabs: the float difference, then double arithmetic
+        if (fabs((double)(error - lasterror)) / (double)error < (double)p->err_rel) return false;
     } else if (p->iter_type == DWHIP_ITER_ABS) {
         if (iter > 0 && error < p->err_abs) return false;
     }

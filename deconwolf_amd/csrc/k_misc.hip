@@ -100,6 +100,7 @@ __global__ __launch_bounds__(STHR) void k_stats_final(const JobDev* __restrict__
     st->mid[1] = (int)((a - (long long)st->mid[2] * pMN) / J.pM);
     st->mid[0] = (int)(a - (long long)st->mid[2] * pMN - (long long)st->mid[1] * J.pM);
     st->iter = 0;
+    st->stop = 0;
 }
 
 // Start guess into U0 and U1 (method_shb.c:264-287).
@@ -124,11 +125,27 @@ __global__ __launch_bounds__(STHR) void k_start(const JobDev* __restrict__ jobs,
 
 // End of a round: error of the iteration from the X-tile partials, in a
 // fixed order, then advance the iteration counter.  One block per job.
+// dw_iterator_next (src/dw.c:237-269) before iteration n, the errors of
+// iterations n-1 and n-2 in J.err: whether iteration n runs.
+__device__ bool iterator_continue_dev(const JobDev& J, int n)
+{
+    const float e = J.err[n - 1];
+    if (J.iter_type == DWHIP_DEV_ITER_REL) {
+        const float last = n < 2 ? 2 * e * J.err_rel : J.err[n - 2];
+        // C's fabs: the float difference, then double arithmetic
+        if (fabs((double)(e - last)) / (double)e < (double)J.err_rel) return false;
+    } else if (J.iter_type == DWHIP_DEV_ITER_ABS) {
+        if (n > 0 && e < J.err_abs) return false;
+    }
+    return n < J.niter;
+}
+
 __global__ __launch_bounds__(STHR) void k_round_end(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs,
                                                     int advance)
 {
     __shared__ double red[STHR / 64];
     const JobDev& J = jobs[blockIdx.x];
+    if (J.dev_stop && J.st->stop) return;  // stopped in an earlier round of this step
     const int nt = J.ntile[KIND_XR];  // one partial per tile of pass E
     // each thread sums its strided partials in index order; the loads of
     // eight of them are issued together
@@ -152,7 +169,10 @@ __global__ __launch_bounds__(STHR) void k_round_end(const BatchDev* __restrict__
             J.err[it] = (float)mean;
             J.errd[it] = mean;
         }
-        if (advance) st->iter = it + 1;
+        if (advance) {
+            st->iter = it + 1;
+            if (J.dev_stop && it < J.niter && !iterator_continue_dev(J, it + 1)) st->stop = 1;
+        }
     }
 }
 

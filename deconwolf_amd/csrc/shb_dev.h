@@ -21,16 +21,19 @@ enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2, KIND_YD = 3, KIND_YF = 4, KIND_XR = 5
 
 // metric / start values mirror dw_metric and dw_start_condition (src/dw.h:64-77)
 enum { DWHIP_DEV_METRIC_MSE = 0, DWHIP_DEV_METRIC_IDIV = 1 };
+// stopping rules, the values of dw_iter_type (src/dw.h) and DWHIP_ITER_*
+enum { DWHIP_DEV_ITER_ABS = 0, DWHIP_DEV_ITER_REL = 1, DWHIP_DEV_ITER_FIXED = 2 };
 
 // State written on the device: by the setup statistics kernel (bg, c0,
-// mid) and by the end-of-round kernel (iter).  JobDev copies in several
-// tables share it through the pointer.
+// mid) and by the end-of-round kernel (iter, stop).  JobDev copies in
+// several tables share it through the pointer.
 struct JobState {
     float bg;           // background level (method_shb.c:108-117)
     float c0;           // flat start constant (method_shb.c:255-268)
     int mid[3];         // PSF argmax (method_shb.c:195)
-    int iter;           // current iteration
-    int pad[2];
+    int iter;           // current iteration (rounds run)
+    int stop;           // 1: the stopping rule ended the job (dev_stop tables)
+    int pad;
 };
 
 struct JobDev {
@@ -52,8 +55,15 @@ struct JobDev {
     int has_w;          // Bertero weights in use (bq > 0)
     int bg_auto;
     int start_identity; // start from start_img (or im) inserted, instead of flat
-    int niter;          // iterations requested
+    int niter;          // iterations requested (nIter, or maxiter for REL/ABS)
     int rl;             // 1: Richardson-Lucy (method_rl.c), 0: SHB (method_shb.c)
+    // stopping rule (dw_iterator_next, src/dw.c:237-269)
+    int iter_type;      // DWHIP_DEV_ITER_*
+    float err_rel, err_abs;
+    // 1 in the served engine's round tables: the end-of-round kernel applies
+    // the stopping rule and sets st->stop, and the passes skip the tiles of
+    // a stopped job (a job's own table keeps 0)
+    int dev_stop;
     float alphamax;
     float bg_user;
     JobState* st;       // device-written state of this job

@@ -785,6 +785,7 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
     const int njobs = bd->njobs;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
+        if (J.dev_stop && gp(J.st)->stop) continue;  // the stopping rule ended this job
         if constexpr (x_fixed_mode(MODE)) {
             if (J.pxf.nst == 2 && J.pxf.radix[0] == 25 && J.pxf.radix[1] == 25) {
                 xpass_tile<MODE, 25, 25>(J, t, threadIdx.x, lds, red);
@@ -933,6 +934,7 @@ void k_colpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
         // XCD-aware order when the grid covers the tiles exactly once
         const int t = (ykind && (int)gridDim.x == total) ? xcd_tile(t0, total) : t0;
         const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
+        if (J.dev_stop && gp(J.st)->stop) continue;  // the stopping rule ended this job
         const int lt = t - J.tile0[kind];
         const int hcs = J.hcs, ny = J.ny;
         const int B = ykind ? J.ycols : J.zcols;

@@ -136,6 +136,62 @@ def test_rel_stopping_in_engine(groups):
     eng.close()
 
 
+@pytest.mark.parametrize("groups", [1, 2])
+def test_device_stopping_multi_round_steps(groups):
+    """REL and ABS jobs of the fused passes stop on the device (the
+    end-of-round kernel applies dw_iterator_next, the passes skip a stopped
+    job), so they run in graph-replayed multi-round steps beside FIXED jobs:
+    each job's iteration count equals its solo run's and its result is
+    bit-identical, and the engine takes fewer steps than rounds."""
+    print("engine: device stopping in multi-round steps, groups", groups, flush=True)
+    jobs = []
+    im, k, s = make_job(31, (10, 24, 30), (5, 7, 7), 1, iter_type=dw.ITER_REL, err_rel=0.05, maxiter=40)
+    jobs.append((im, k, s))
+    im, k, s = make_job(32, (12, 30, 28), (5, 7, 7), 1, iter_type=dw.ITER_REL, err_rel=0.01, maxiter=40,
+                        metric=dw.METRIC_MSE)
+    jobs.append((im, k, s))
+    im, k, s = make_job(33, (10, 26, 32), (5, 5, 5), 1, iter_type=dw.ITER_REL, err_rel=0.02, maxiter=40,
+                        method=dw.METHOD_RL)
+    jobs.append((im, k, s))
+    # ABS: a bound crossed within the run (from a fixed run's error trace)
+    im, k, s = make_job(34, (12, 28, 36), (7, 7, 7), 25)
+    _, sf = solo(im, k, s)
+    tr = sf.err_trace
+    s.iter_type, s.err_abs, s.maxiter = dw.ITER_ABS, float(tr[9]) * 1.0001, 40
+    jobs.append((im, k, s))
+    # REL stopped by maxiter
+    im, k, s = make_job(35, (8, 20, 24), (5, 5, 5), 1, iter_type=dw.ITER_REL, err_rel=1e-9, maxiter=11)
+    jobs.append((im, k, s))
+    im, k, s = make_job(36, (16, 36, 40), (7, 9, 9), 17)
+    jobs.append((im, k, s))
+    def alone(im, k, s):
+        if s.method != dw.METHOD_RL:
+            return solo(im, k, s)
+        import copy
+        s2 = copy.copy(s)
+        out = dw.deconvolve_rl_hip_device(im, k, s2)
+        torch.cuda.synchronize()
+        return out.cpu().numpy(), s2
+
+    refs = [alone(im, k, s) for im, k, s in jobs]
+    eng = Engine(hbm_budget_gb=2.0, max_active=16, groups=groups, rounds_per_step=8)
+    for i, (im, k, s) in enumerate(jobs):
+        eng.submit(i, im, k, s)
+    done, steps = [], 0
+    while len(done) < len(jobs):
+        done += eng.step()
+        steps += 1
+        assert steps < 200
+    rounds = max(r[1].stats["iterations"] for r in refs)
+    for i, (ref, s2) in enumerate(refs):
+        out, st = eng.result(i)
+        assert st["iterations"] == s2.stats["iterations"], (i, st["iterations"], s2.stats["iterations"])
+        assert np.array_equal(out.cpu().numpy(), ref), i
+    assert refs[3][1].stats["iterations"] < 25 and refs[4][1].stats["iterations"] == 11
+    assert steps < rounds, (steps, rounds)
+    eng.close()
+
+
 def test_shortest_first_and_voxel_cap():
     """ORDER_SHORTEST admits the least work first; active_mvox caps the
     transform voxels iterating together (a job above the cap runs alone);
