@@ -219,10 +219,12 @@ int ntiles_axis(const GenDev& g, int a)
     return nxb * (a == 1 ? g.nz : g.ny);
 }
 
+// a: axis 0-2, or 3 / 4 for the z axis with the K / conj(K) junction
 int axis(const DwhipJob* j, int a, int sign, hipStream_t s)
 {
-    const int grid = std::min(ntiles_axis(j->g, a), 65536);
-    const hipError_t e = dwhip_gen_launch_axis(j->gd, a, sign, grid, dwhip_generic_lds(j, a), s);
+    const int ax = std::min(a, 2);
+    const int grid = std::min(ntiles_axis(j->g, ax), 65536);
+    const hipError_t e = dwhip_gen_launch_axis(j->gd, a, sign, grid, dwhip_generic_lds(j, ax), s);
     return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "any-size axis pass", __LINE__);
 }
 
@@ -239,12 +241,15 @@ int elem(const DwhipJob* j, int op, hipStream_t s)
         if (r_ != DWHIP_OK) return r_; \
     } while (0)
 
-// Z <- IFFT(op(K) FFT(Z)), unnormalised (1/(nx ny nz) is in K)
+// Z <- IFFT(op(K) FFT(Z)), unnormalised (1/(nx ny nz) is in K); the z
+// transforms and the K multiply in one pass
 int convolve(const DwhipJob* j, bool conj, hipStream_t s)
 {
-    for (int a = 0; a < 3; ++a) GCHK(axis(j, a, -1, s));
-    GCHK(elem(j, conj ? OP_MULKC : OP_MULK, s));
-    for (int a = 2; a >= 0; --a) GCHK(axis(j, a, +1, s));
+    GCHK(axis(j, 0, -1, s));
+    GCHK(axis(j, 1, -1, s));
+    GCHK(axis(j, conj ? 4 : 3, 0, s));
+    GCHK(axis(j, 1, +1, s));
+    GCHK(axis(j, 0, +1, s));
     return DWHIP_OK;
 }
 

@@ -180,9 +180,31 @@ __device__ __forceinline__ int gntiles(const GenDev& G, int B)
     return nxb * (AXIS == 1 ? G.nz : G.ny);
 }
 
+// Length-n transform of the nb pencils in `a` (the pre-chirp applied when
+// Bluestein), `t` scratch; returns the buffer holding the result (before
+// the Bluestein post-chirp).
+__device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const float2* __restrict__ tw, int nb,
+                              int sign)
+{
+    float2* r = gfft(a, t, pl, tw, nb, pl.bluestein ? -1 : sign);
+    if (pl.bluestein) {
+        const float2* H = tw + (sign < 0 ? pl.hf : pl.hi);
+        for (int idx = threadIdx.x; idx < nb * pl.m; idx += GTHR) {
+            const int k = idx % pl.m;
+            r[idx] = gmul(r[idx], H[k]);
+        }
+        __syncthreads();
+        r = gfft(r, r == a ? t : a, pl, tw, nb, +1);
+    }
+    return r;
+}
+
 // In-place complex FFT of every pencil of Z along AXIS; sign -1 forward,
-// +1 inverse (unnormalised).
-template <int AXIS>
+// +1 inverse (unnormalised).  JUNC > 0 (the z axis): forward transform,
+// times K (JUNC 1, fft_mul_inplace, fft.c:437-453) or conj(K) (JUNC 2,
+// fft_mul_conj_inplace, fft.c:490-512), inverse transform, in LDS: the
+// junction of a convolution in one pass instead of three.
+template <int AXIS, int JUNC>
 __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, int sign)
 {
     extern __shared__ float2 sh[];
@@ -193,8 +215,9 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
     float2* t = sh + (size_t)B * m;
     const float2* __restrict__ tw = G.tw;
     const float2* chirp = tw + pl.chirp;
-    const float2* H = tw + (sign < 0 ? pl.hf : pl.hi);
+    const int s_in = JUNC ? -1 : sign, s_out = JUNC ? +1 : sign;
     float2* __restrict__ Z = G.Z;
+    const float2* __restrict__ K = G.K;
     const int ntile = gntiles<AXIS>(G, B);
     for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         size_t base, pstep, estep;
@@ -208,26 +231,38 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
             float2 v = make_float2(0.f, 0.f);
             if (b < nb && i < n) {
                 v = Z[base + b * pstep + i * estep];
-                if (pl.bluestein) v = sign < 0 ? gmul(v, chirp[i]) : gmulc(v, chirp[i]);
+                if (pl.bluestein) v = s_in < 0 ? gmul(v, chirp[i]) : gmulc(v, chirp[i]);
             }
             a[b * m + i] = v;
         }
         __syncthreads();
-        float2* r = gfft(a, t, pl, tw, nb, pl.bluestein ? -1 : sign);
-        if (pl.bluestein) {
-            for (int idx = threadIdx.x; idx < nb * m; idx += GTHR) {
-                const int k = idx % m;
-                r[idx] = gmul(r[idx], H[k]);
+        float2* r = gtransform(a, t, pl, tw, nb, s_in);
+        if (JUNC) {
+            // spectrum (post-chirp) times K, then the inverse's pre-chirp,
+            // into the other buffer (zero beyond n, as the load step pads)
+            float2* o = r == a ? t : a;
+            for (int idx = threadIdx.x; idx < B * m; idx += GTHR) {
+                const int b = AXIS == 0 ? idx / m : idx % B;
+                const int i = AXIS == 0 ? idx - b * m : idx / B;
+                float2 v = make_float2(0.f, 0.f);
+                if (b < nb && i < n) {
+                    v = r[b * m + i];
+                    if (pl.bluestein) v = gmul(v, chirp[i]);
+                    const float2 k = K[base + b * pstep + i * estep];
+                    v = JUNC == 2 ? gmulc(v, k) : gmul(v, k);
+                    if (pl.bluestein) v = gmulc(v, chirp[i]);
+                }
+                o[b * m + i] = v;
             }
             __syncthreads();
-            r = gfft(r, r == a ? t : a, pl, tw, nb, +1);
+            r = gtransform(o, o == a ? t : a, pl, tw, nb, s_out);
         }
         for (int idx = threadIdx.x; idx < B * n; idx += GTHR) {
             const int b = AXIS == 0 ? idx / n : idx % B;
             const int i = AXIS == 0 ? idx - b * n : idx / B;
             if (b >= nb) continue;
             float2 v = r[b * m + i];
-            if (pl.bluestein) v = sign < 0 ? gmul(v, chirp[i]) : gmulc(v, chirp[i]);
+            if (pl.bluestein) v = s_out < 0 ? gmul(v, chirp[i]) : gmulc(v, chirp[i]);
             Z[base + b * pstep + i * estep] = v;
         }
         __syncthreads();
@@ -394,10 +429,14 @@ extern "C" {
 
 hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid, size_t lds, hipStream_t s)
 {
+    // axis 3 / 4: the z axis with the K / conj(K) junction
     switch (axis) {
-    case 0: hipLaunchKernelGGL(gk_axis<0>, dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
-    case 1: hipLaunchKernelGGL(gk_axis<1>, dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
-    default: hipLaunchKernelGGL(gk_axis<2>, dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    case 0: hipLaunchKernelGGL((gk_axis<0, 0>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    case 1: hipLaunchKernelGGL((gk_axis<1, 0>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    case 2: hipLaunchKernelGGL((gk_axis<2, 0>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    case 3: hipLaunchKernelGGL((gk_axis<2, 1>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    case 4: hipLaunchKernelGGL((gk_axis<2, 2>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -405,11 +444,11 @@ hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid,
 // Raises the dynamic-LDS limit of the axis kernels (once per process).
 hipError_t dwhip_gen_lds_limit(size_t bytes)
 {
-    hipError_t e = hipFuncSetAttribute((const void*)gk_axis<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)gk_axis<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)gk_axis<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const void* fs[5] = {(const void*)gk_axis<0, 0>, (const void*)gk_axis<1, 0>, (const void*)gk_axis<2, 0>,
+                         (const void*)gk_axis<2, 1>, (const void*)gk_axis<2, 2>};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 5 && e == hipSuccess; ++i)
+        e = hipFuncSetAttribute(fs[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     return e;
 }
 
