@@ -29,8 +29,15 @@ namespace {
 
 enum { OP_PSF = 0, OP_OTF, OP_ONES, OP_WEIGHTS, OP_MULK, OP_MULKC, OP_UPDATE, OP_RATIO_IDIV, OP_RATIO_MSE };
 
-// LDS for the two ping-pong buffers of an axis workgroup
+// LDS for the two ping-pong buffers of an axis workgroup: at most kGenLds,
+// and pencils per workgroup chosen within kGenWgLds (a build tunable, KiB),
+// so that several workgroups share a CU: the axis kernels are bound by
+// latency more than by bytes (120 VGPRs, up to four waves per SIMD)
+#ifndef DWHIP_GEN_WG_LDS_KB
+#define DWHIP_GEN_WG_LDS_KB 75
+#endif
 constexpr size_t kGenLds = 150 * 1024;
+constexpr size_t kGenWgLds = DWHIP_GEN_WG_LDS_KB * 1024;
 constexpr int kElemGrid = 4096;    // grid of the elementwise kernels
 constexpr int kRatioGrid = 2048;   // fixed grid of the ratio kernel: its partial sums
 
@@ -148,7 +155,7 @@ int dwhip_generic_plan(DwhipJob* j)
         }
         // pencils per workgroup: as many as the LDS holds, at most 16
         // (adjacent x columns of the y and z passes share their lines)
-        const int bmax = (int)(kGenLds / (2 * (size_t)m * sizeof(float2)));
+        const int bmax = (int)(kGenWgLds / (2 * (size_t)m * sizeof(float2)));
         p.B = std::max(1, std::min(16, bmax));
         p.chirp = p.hf = p.hi = 0;
         if (blue) {

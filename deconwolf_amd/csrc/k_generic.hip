@@ -39,76 +39,16 @@ __device__ __forceinline__ float g_extrap(float x, float xp, double alpha, float
     return p < bg ? bg : p;
 }
 
-// cos and sin of 2 pi q / R for R in {3, 5, 7}
-template <int R>
-__device__ __forceinline__ void groot(int q, float* c, float* s)
-{
-    const int h = q <= R / 2 ? q : R - q;
-    const float sg = q <= R / 2 ? 1.f : -1.f;
-    float cc = 1.f, ss = 0.f;
-    if (R == 3) {
-        if (h == 1) { cc = -0.5f; ss = 0.8660254037844386f; }
-    } else if (R == 5) {
-        if (h == 1) { cc = 0.30901699437494745f; ss = 0.9510565162951535f; }
-        if (h == 2) { cc = -0.8090169943749475f; ss = 0.5877852522924731f; }
-    } else if (R == 7) {
-        if (h == 1) { cc = 0.6234898018587336f; ss = 0.7818314824680298f; }
-        if (h == 2) { cc = -0.22252093395631434f; ss = 0.9749279121818236f; }
-        if (h == 3) { cc = -0.9009688679024191f; ss = 0.4338837391175582f; }
-    }
-    *c = cc;
-    *s = sg * ss;
-}
-
-// y_q = sum_t v_t exp(sign 2 pi i t q / R) for R in {2, 3, 4, 5, 7}
-template <int R>
-__device__ __forceinline__ void gdft(float2* v, int sign)
-{
-    if (R == 2) {
-        const float2 a = v[0], b = v[1];
-        v[0] = make_float2(a.x + b.x, a.y + b.y);
-        v[1] = make_float2(a.x - b.x, a.y - b.y);
-    } else if (R == 4) {
-        const float2 a = v[0], b = v[1], c = v[2], d = v[3];
-        const float2 s02 = make_float2(a.x + c.x, a.y + c.y), d02 = make_float2(a.x - c.x, a.y - c.y);
-        const float2 s13 = make_float2(b.x + d.x, b.y + d.y), d13 = make_float2(b.x - d.x, b.y - d.y);
-        // (sign i) * d13
-        const float2 r = sign < 0 ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
-        v[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
-        v[1] = make_float2(d02.x + r.x, d02.y + r.y);
-        v[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
-        v[3] = make_float2(d02.x - r.x, d02.y - r.y);
-    } else {
-        float cs[R], sn[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            groot<R>(q, &cs[q], &sn[q]);
-            sn[q] *= (float)sign;
-        }
-        float2 y[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            float2 acc = v[0];
-#pragma unroll
-            for (int t = 1; t < R; ++t) {
-                const int e = (t * q) % R;
-                acc.x += v[t].x * cs[e] - v[t].y * sn[e];
-                acc.y += v[t].x * sn[e] + v[t].y * cs[e];
-            }
-            y[q] = acc;
-        }
-#pragma unroll
-        for (int q = 0; q < R; ++q) v[q] = y[q];
-    }
-}
-
 template <int R>
 __device__ __forceinline__ void gstage(const float2* __restrict__ src, float2* __restrict__ dst, int m, int nb,
                                        int ns, const float2* __restrict__ tw, int sign)
 {
     const int L = m / R;
+    // indices by magic multiply (fft_engine.h's fdiv; j < 2^24, so exact)
+    const uint32_t Lm = fdiv_magic((uint32_t)L), nsm = fdiv_magic((uint32_t)ns);
     for (int j = threadIdx.x; j < nb * L; j += GTHR) {
-        const int b = j / L, jj = j - b * L, k = jj % ns;
+        const int b = (int)fdiv((uint32_t)j, Lm), jj = j - b * L;
+        const int k = jj - (int)fdiv((uint32_t)jj, nsm) * ns;
         float2 v[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) v[t] = src[b * m + jj + t * L];
@@ -119,7 +59,11 @@ __device__ __forceinline__ void gstage(const float2* __restrict__ src, float2* _
                 v[t] = sign < 0 ? gmul(v[t], w) : gmulc(v[t], w);
             }
         }
-        gdft<R>(v, sign);
+        // the fused passes' small DFTs (fft_engine.h)
+        if (sign < 0)
+            Dft<R, -1>::run(v);
+        else
+            Dft<R, +1>::run(v);
         const int o = b * m + (jj - k) * R + k;
 #pragma unroll
         for (int t = 0; t < R; ++t) dst[o + t * ns] = v[t];
@@ -219,7 +163,15 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
     float2* __restrict__ Z = G.Z;
     const float2* __restrict__ K = G.K;
     const int ntile = gntiles<AXIS>(G, B);
-    for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    for (int t0 = blockIdx.x; t0 < ntile; t0 += gridDim.x) {
+        // y and z: the tiles that share a row's lines (adjacent x columns)
+        // on one XCD, consecutive there (workgroup b runs on XCD b mod 8),
+        // when the grid covers the tiles once
+        int tile = t0;
+        if (AXIS != 0 && (int)gridDim.x == ntile) {
+            const int q = ntile >> 3, rr = ntile & 7, x = t0 & 7, kk = t0 >> 3;
+            tile = x * q + min(x, rr) + kk;
+        }
         size_t base, pstep, estep;
         int nb;
         gpencils<AXIS>(G, B, tile, &base, &pstep, &estep, &nb);
