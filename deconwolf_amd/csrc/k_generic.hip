@@ -133,8 +133,9 @@ __device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const 
     float2* r = gfft(a, t, pl, tw, nb, pl.bluestein ? -1 : sign);
     if (pl.bluestein) {
         const float2* H = tw + (sign < 0 ? pl.hf : pl.hi);
+        const uint32_t mm = fdiv_magic((uint32_t)pl.m);
         for (int idx = threadIdx.x; idx < nb * pl.m; idx += GTHR) {
-            const int k = idx % pl.m;
+            const int k = idx - (int)fdiv((uint32_t)idx, mm) * pl.m;
             r[idx] = gmul(r[idx], H[k]);
         }
         __syncthreads();
@@ -163,6 +164,8 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
     float2* __restrict__ Z = G.Z;
     const float2* __restrict__ K = G.K;
     const int ntile = gntiles<AXIS>(G, B);
+    // index splits by magic multiply (fft_engine.h's fdiv; indices < 2^24)
+    const uint32_t mmag = fdiv_magic((uint32_t)m), nmag = fdiv_magic((uint32_t)n), bmag = fdiv_magic((uint32_t)B);
     for (int t0 = blockIdx.x; t0 < ntile; t0 += gridDim.x) {
         // y and z: the tiles that share a row's lines (adjacent x columns)
         // on one XCD, consecutive there (workgroup b runs on XCD b mod 8),
@@ -178,8 +181,9 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
         for (int idx = threadIdx.x; idx < B * m; idx += GTHR) {
             // x rows: element-fastest (consecutive addresses); y/z columns:
             // pencil-fastest (adjacent x, consecutive addresses)
-            const int b = AXIS == 0 ? idx / m : idx % B;
-            const int i = AXIS == 0 ? idx - b * m : idx / B;
+            const int q = (int)fdiv((uint32_t)idx, AXIS == 0 ? mmag : bmag);
+            const int b = AXIS == 0 ? q : idx - q * B;
+            const int i = AXIS == 0 ? idx - q * m : q;
             float2 v = make_float2(0.f, 0.f);
             if (b < nb && i < n) {
                 v = Z[base + b * pstep + i * estep];
@@ -194,8 +198,9 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
             // into the other buffer (zero beyond n, as the load step pads)
             float2* o = r == a ? t : a;
             for (int idx = threadIdx.x; idx < B * m; idx += GTHR) {
-                const int b = AXIS == 0 ? idx / m : idx % B;
-                const int i = AXIS == 0 ? idx - b * m : idx / B;
+                const int q = (int)fdiv((uint32_t)idx, AXIS == 0 ? mmag : bmag);
+                const int b = AXIS == 0 ? q : idx - q * B;
+                const int i = AXIS == 0 ? idx - q * m : q;
                 float2 v = make_float2(0.f, 0.f);
                 if (b < nb && i < n) {
                     v = r[b * m + i];
@@ -210,8 +215,9 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
             r = gtransform(o, o == a ? t : a, pl, tw, nb, s_out);
         }
         for (int idx = threadIdx.x; idx < B * n; idx += GTHR) {
-            const int b = AXIS == 0 ? idx / n : idx % B;
-            const int i = AXIS == 0 ? idx - b * n : idx / B;
+            const int q = (int)fdiv((uint32_t)idx, AXIS == 0 ? nmag : bmag);
+            const int b = AXIS == 0 ? q : idx - q * B;
+            const int i = AXIS == 0 ? idx - q * n : q;
             if (b >= nb) continue;
             float2 v = r[b * m + i];
             if (pl.bluestein) v = s_out < 0 ? gmul(v, chirp[i]) : gmulc(v, chirp[i]);
