@@ -353,6 +353,33 @@ def run_c5(fields=1, chunk=4):
     return res
 
 
+def run_anysize():
+    """c2 with --bq 1: deconwolf's exact work size (1115 = 5 * 223 along x
+    and y, Bluestein there), timed at 3 and 13 iterations (best of two
+    each); the difference gives the time per iteration."""
+    import torch
+    import deconwolf_amd as dw
+    from deconwolf_amd import workloads
+
+    im, psf, _, _ = workloads.make("c2")
+    imt = torch.from_numpy(im).to("cuda:0")
+    pst = torch.from_numpy(psf).to("cuda:0")
+    best = {}
+    for n in (3, 13, 3, 13):
+        s = job_opts(nIter=n, borderQuality=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dw.deconvolve_shb_hip_device(imt, pst, s)
+        torch.cuda.synchronize()
+        best[n] = min(best.get(n, 1e9), time.perf_counter() - t0)
+    it_ms = 1e3 * (best[13] - best[3]) / 10
+    M, N, P = im.shape[2], im.shape[1], im.shape[0]
+    return {"workload": "c2 at border quality 1: 1024x1024x60, PSF 181x181x79, work size %s (any-size path)" %
+                        "x".join(str(v) for v in s.stats["work"]),
+            "job_3_iterations_s": best[3], "job_13_iterations_s": best[13], "ms_per_iteration": it_ms,
+            "image_voxel_iter_per_s": M * N * P / (it_ms * 1e-3)}
+
+
 def main():
     global ENGINE_GROUPS, SERVE_AGING_MS, PLANNING, ROUNDS_PER_STEP, STREAM_REL
     ap = argparse.ArgumentParser()
@@ -390,6 +417,9 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy a GIB-sized float32 buffer on the GPU 3 times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
+    ap.add_argument("--anysize", action="store_true",
+                    help="only time the c2 job at border quality 1 (deconwolf's exact work size 1115x1115x100, "
+                         "the any-size path) at 3 and 13 iterations, print one JSON line and exit")
     a = ap.parse_args()
     ENGINE_GROUPS = a.engine_groups
     SERVE_AGING_MS = a.aging_ms
@@ -405,6 +435,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 3}))
+        return
+    if a.anysize:
+        print(json.dumps(run_anysize()))
         return
     if a.sweep:
         if a.gpus != 1:
