@@ -30,11 +30,12 @@ namespace {
 enum { OP_PSF = 0, OP_OTF, OP_ONES, OP_WEIGHTS, OP_MULK, OP_MULKC, OP_UPDATE, OP_RATIO_IDIV, OP_RATIO_MSE };
 
 // LDS for the two ping-pong buffers of an axis workgroup: at most kGenLds,
-// and pencils per workgroup chosen within kGenWgLds (a build tunable, KiB),
-// so that several workgroups share a CU: the axis kernels are bound by
-// latency more than by bytes (120 VGPRs, up to four waves per SIMD)
+// and pencils per workgroup chosen within kGenWgLds (a build tunable, KiB).
+// The axis kernels are bound by latency more than by bytes: one workgroup
+// of 1024 threads per CU (120 VGPRs, four waves per SIMD) with the whole
+// budget, or with 256 threads two workgroups of 75 KiB each.
 #ifndef DWHIP_GEN_WG_LDS_KB
-#define DWHIP_GEN_WG_LDS_KB 75
+#define DWHIP_GEN_WG_LDS_KB 150
 #endif
 constexpr size_t kGenLds = 150 * 1024;
 constexpr size_t kGenWgLds = DWHIP_GEN_WG_LDS_KB * 1024;

@@ -10,7 +10,14 @@
 
 #include "generic_dev.h"
 
-#define GTHR 256
+#define GTHR 256  // threads per workgroup, elementwise kernels
+// threads per workgroup of the axis kernels (a build tunable): their 120
+// VGPRs allow 1024, four waves per SIMD (c2 at border quality 1: 41.4 ms
+// per iteration against 42.7 with 256 threads and two workgroups per CU)
+#ifndef DWHIP_GEN_AXIS_THREADS
+#define DWHIP_GEN_AXIS_THREADS 1024
+#endif
+#define GATHR DWHIP_GEN_AXIS_THREADS
 
 namespace {
 
@@ -46,7 +53,7 @@ __device__ __forceinline__ void gstage(const float2* __restrict__ src, float2* _
     const int L = m / R;
     // indices by magic multiply (fft_engine.h's fdiv; j < 2^24, so exact)
     const uint32_t Lm = fdiv_magic((uint32_t)L), nsm = fdiv_magic((uint32_t)ns);
-    for (int j = threadIdx.x; j < nb * L; j += GTHR) {
+    for (int j = threadIdx.x; j < nb * L; j += GATHR) {
         const int b = (int)fdiv((uint32_t)j, Lm), jj = j - b * L;
         const int k = jj - (int)fdiv((uint32_t)jj, nsm) * ns;
         float2 v[R];
@@ -134,7 +141,7 @@ __device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const 
     if (pl.bluestein) {
         const float2* H = tw + (sign < 0 ? pl.hf : pl.hi);
         const uint32_t mm = fdiv_magic((uint32_t)pl.m);
-        for (int idx = threadIdx.x; idx < nb * pl.m; idx += GTHR) {
+        for (int idx = threadIdx.x; idx < nb * pl.m; idx += GATHR) {
             const int k = idx - (int)fdiv((uint32_t)idx, mm) * pl.m;
             r[idx] = gmul(r[idx], H[k]);
         }
@@ -150,7 +157,7 @@ __device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const 
 // fft_mul_conj_inplace, fft.c:490-512), inverse transform, in LDS: the
 // junction of a convolution in one pass instead of three.
 template <int AXIS, int JUNC>
-__global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, int sign)
+__global__ __launch_bounds__(GATHR) void gk_axis(const GenDev* __restrict__ gd, int sign)
 {
     extern __shared__ float2 sh[];
     const GenDev& G = *gd;
@@ -178,7 +185,7 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
         size_t base, pstep, estep;
         int nb;
         gpencils<AXIS>(G, B, tile, &base, &pstep, &estep, &nb);
-        for (int idx = threadIdx.x; idx < B * m; idx += GTHR) {
+        for (int idx = threadIdx.x; idx < B * m; idx += GATHR) {
             // x rows: element-fastest (consecutive addresses); y/z columns:
             // pencil-fastest (adjacent x, consecutive addresses)
             const int q = (int)fdiv((uint32_t)idx, AXIS == 0 ? mmag : bmag);
@@ -197,7 +204,7 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
             // spectrum (post-chirp) times K, then the inverse's pre-chirp,
             // into the other buffer (zero beyond n, as the load step pads)
             float2* o = r == a ? t : a;
-            for (int idx = threadIdx.x; idx < B * m; idx += GTHR) {
+            for (int idx = threadIdx.x; idx < B * m; idx += GATHR) {
                 const int q = (int)fdiv((uint32_t)idx, AXIS == 0 ? mmag : bmag);
                 const int b = AXIS == 0 ? q : idx - q * B;
                 const int i = AXIS == 0 ? idx - q * m : q;
@@ -214,7 +221,7 @@ __global__ __launch_bounds__(GTHR) void gk_axis(const GenDev* __restrict__ gd, i
             __syncthreads();
             r = gtransform(o, o == a ? t : a, pl, tw, nb, s_out);
         }
-        for (int idx = threadIdx.x; idx < B * n; idx += GTHR) {
+        for (int idx = threadIdx.x; idx < B * n; idx += GATHR) {
             const int q = (int)fdiv((uint32_t)idx, AXIS == 0 ? nmag : bmag);
             const int b = AXIS == 0 ? q : idx - q * B;
             const int i = AXIS == 0 ? idx - q * n : q;
@@ -389,11 +396,11 @@ hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid,
 {
     // axis 3 / 4: the z axis with the K / conj(K) junction
     switch (axis) {
-    case 0: hipLaunchKernelGGL((gk_axis<0, 0>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
-    case 1: hipLaunchKernelGGL((gk_axis<1, 0>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
-    case 2: hipLaunchKernelGGL((gk_axis<2, 0>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
-    case 3: hipLaunchKernelGGL((gk_axis<2, 1>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
-    case 4: hipLaunchKernelGGL((gk_axis<2, 2>), dim3(grid), dim3(GTHR), lds, s, gd, sign); break;
+    case 0: hipLaunchKernelGGL((gk_axis<0, 0>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
+    case 1: hipLaunchKernelGGL((gk_axis<1, 0>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
+    case 2: hipLaunchKernelGGL((gk_axis<2, 0>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
+    case 3: hipLaunchKernelGGL((gk_axis<2, 1>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
+    case 4: hipLaunchKernelGGL((gk_axis<2, 2>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
