@@ -17,7 +17,8 @@
 #include "generic_dev.h"
 
 extern "C" {
-hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid, size_t lds, hipStream_t s);
+hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid, size_t lds, int threads,
+                                 hipStream_t s);
 hipError_t dwhip_gen_launch_elem(const GenDev* gd, int op, int grid, hipStream_t s);
 hipError_t dwhip_gen_lds_limit(size_t bytes);
 hipError_t dwhip_launch_stats(const JobDev* jobs, int j, int nblk, hipStream_t s);
@@ -31,9 +32,9 @@ enum { OP_PSF = 0, OP_OTF, OP_ONES, OP_WEIGHTS, OP_MULK, OP_MULKC, OP_UPDATE, OP
 
 // LDS for the two ping-pong buffers of an axis workgroup: at most kGenLds,
 // and pencils per workgroup chosen within kGenWgLds (a build tunable, KiB).
-// The axis kernels are bound by latency more than by bytes: one workgroup
-// of 1024 threads per CU (120 VGPRs, four waves per SIMD) with the whole
-// budget, or with 256 threads two workgroups of 75 KiB each.
+// The axis kernels are bound by latency more than by bytes: long pencils
+// (Bluestein lengths) run one workgroup of 1024 threads per CU with the
+// whole budget, short ones several workgroups of 256 threads.
 #ifndef DWHIP_GEN_WG_LDS_KB
 #define DWHIP_GEN_WG_LDS_KB 150
 #endif
@@ -232,7 +233,10 @@ int axis(const DwhipJob* j, int a, int sign, hipStream_t s)
 {
     const int ax = std::min(a, 2);
     const int grid = std::min(ntiles_axis(j->g, ax), 65536);
-    const hipError_t e = dwhip_gen_launch_axis(j->gd, a, sign, grid, dwhip_generic_lds(j, ax), s);
+    // threads for about eight values each of the tile's B*m (at least 256)
+    const int vals = j->g.plan[ax].B * j->g.plan[ax].m;
+    const int threads = vals >= 8 * 1024 ? 1024 : vals >= 8 * 512 ? 512 : 256;
+    const hipError_t e = dwhip_gen_launch_axis(j->gd, a, sign, grid, dwhip_generic_lds(j, ax), threads, s);
     return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "any-size axis pass", __LINE__);
 }
 

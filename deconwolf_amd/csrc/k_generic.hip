@@ -11,13 +11,12 @@
 #include "generic_dev.h"
 
 #define GTHR 256  // threads per workgroup, elementwise kernels
-// threads per workgroup of the axis kernels (a build tunable): their 120
-// VGPRs allow 1024, four waves per SIMD (c2 at border quality 1: 41.4 ms
-// per iteration against 42.7 with 256 threads and two workgroups per CU)
-#ifndef DWHIP_GEN_AXIS_THREADS
-#define DWHIP_GEN_AXIS_THREADS 1024
-#endif
-#define GATHR DWHIP_GEN_AXIS_THREADS
+// The axis kernels are compiled for up to 1024 threads per workgroup (120
+// VGPRs, four waves per SIMD) and launched with as many as their tile
+// needs (256, 512 or 1024: the host's choice per axis); their loops stride
+// by the launch's blockDim.x.
+#define GATHR_MAX 1024
+#define GATHR ((int)blockDim.x)
 
 namespace {
 
@@ -157,7 +156,7 @@ __device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const 
 // fft_mul_conj_inplace, fft.c:490-512), inverse transform, in LDS: the
 // junction of a convolution in one pass instead of three.
 template <int AXIS, int JUNC>
-__global__ __launch_bounds__(GATHR) void gk_axis(const GenDev* __restrict__ gd, int sign)
+__global__ __launch_bounds__(GATHR_MAX) void gk_axis(const GenDev* __restrict__ gd, int sign)
 {
     extern __shared__ float2 sh[];
     const GenDev& G = *gd;
@@ -392,15 +391,17 @@ __global__ __launch_bounds__(GTHR) void gk_ratio(const GenDev* __restrict__ gd, 
 
 extern "C" {
 
-hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid, size_t lds, hipStream_t s)
+hipError_t dwhip_gen_launch_axis(const GenDev* gd, int axis, int sign, int grid, size_t lds, int threads,
+                                 hipStream_t s)
 {
+    if (threads != 256 && threads != 512 && threads != GATHR_MAX) return hipErrorInvalidValue;
     // axis 3 / 4: the z axis with the K / conj(K) junction
     switch (axis) {
-    case 0: hipLaunchKernelGGL((gk_axis<0, 0>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
-    case 1: hipLaunchKernelGGL((gk_axis<1, 0>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
-    case 2: hipLaunchKernelGGL((gk_axis<2, 0>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
-    case 3: hipLaunchKernelGGL((gk_axis<2, 1>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
-    case 4: hipLaunchKernelGGL((gk_axis<2, 2>), dim3(grid), dim3(GATHR), lds, s, gd, sign); break;
+    case 0: hipLaunchKernelGGL((gk_axis<0, 0>), dim3(grid), dim3(threads), lds, s, gd, sign); break;
+    case 1: hipLaunchKernelGGL((gk_axis<1, 0>), dim3(grid), dim3(threads), lds, s, gd, sign); break;
+    case 2: hipLaunchKernelGGL((gk_axis<2, 0>), dim3(grid), dim3(threads), lds, s, gd, sign); break;
+    case 3: hipLaunchKernelGGL((gk_axis<2, 1>), dim3(grid), dim3(threads), lds, s, gd, sign); break;
+    case 4: hipLaunchKernelGGL((gk_axis<2, 2>), dim3(grid), dim3(threads), lds, s, gd, sign); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
