@@ -155,9 +155,15 @@ int dwhip_generic_plan(DwhipJob* j)
                 }
             ns *= r[s];
         }
-        // pencils per workgroup: as many as the LDS holds, at most 16
-        // (adjacent x columns of the y and z passes share their lines)
-        const int bmax = (int)(kGenWgLds / (2 * (size_t)m * sizeof(float2)));
+        p.twlen = p.nst > 0 ? (int)tw.size() - p.twoff[0] : 0;
+        // pencils per workgroup: as many as the LDS holds beside the stage
+        // twiddles (read there rather than from memory in every butterfly:
+        // the stages are latency-bound), at most 16 (adjacent x columns of
+        // the y and z passes share their lines); without the twiddles when
+        // they do not fit beside one pencil
+        const size_t pen = 2 * (size_t)m * sizeof(float2), twb = (size_t)p.twlen * sizeof(float2);
+        p.twlds = pen + twb <= kGenWgLds ? 1 : 0;
+        const int bmax = (int)((kGenWgLds - (p.twlds ? twb : 0)) / pen);
         p.B = std::max(1, std::min(16, bmax));
         p.chirp = p.hf = p.hi = 0;
         if (blue) {
@@ -190,7 +196,7 @@ int dwhip_generic_plan(DwhipJob* j)
 size_t dwhip_generic_lds(const DwhipJob* j, int axis)
 {
     const GenPlanDev& p = j->g.plan[axis];
-    return 2 * (size_t)p.B * p.m * sizeof(float2);
+    return 2 * (size_t)p.B * p.m * sizeof(float2) + (p.twlds ? (size_t)p.twlen * sizeof(float2) : 0);
 }
 
 int dwhip_generic_ratio_grid() { return kRatioGrid; }

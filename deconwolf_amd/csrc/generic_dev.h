@@ -21,6 +21,8 @@ struct GenPlanDev {
     int radix[GEN_MAX_STAGES];
     int ns[GEN_MAX_STAGES];     // product of the radices before the stage
     int twoff[GEN_MAX_STAGES];  // twiddles of the stage (forward sign) in tw
+    int twlen;       // stage twiddles in all, contiguous from twoff[0]
+    int twlds;       // 1: a workgroup copies them to LDS beside its pencils
     int chirp;       // n values c_j = exp(-i pi j^2 / n) in tw (Bluestein)
     int hf, hi;      // m values each: FFT_m(conj(c) / m) and FFT_m(c / m), circularly extended
 };

@@ -77,12 +77,14 @@ __device__ __forceinline__ void gstage(const float2* __restrict__ src, float2* _
 }
 
 // Unnormalised length-m FFT of nb pencils (pencil b at a + b*m), ping-pong
-// between a and t; returns the buffer holding the result.
-__device__ float2* gfft(float2* a, float2* t, const GenPlanDev& pl, const float2* __restrict__ tw, int nb, int sign)
+// between a and t; returns the buffer holding the result.  stw: the stage
+// twiddles (stage s at stw + twoff[s] - twoff[0]), in LDS or memory.
+__device__ __forceinline__ float2* gfft(float2* a, float2* t, const GenPlanDev& pl, const float2* __restrict__ stw,
+                                        int nb, int sign)
 {
     float2 *src = a, *dst = t;
     for (int s = 0; s < pl.nst; ++s) {
-        const float2* tws = tw + pl.twoff[s];
+        const float2* tws = stw + (pl.twoff[s] - pl.twoff[0]);
         switch (pl.radix[s]) {
         case 2: gstage<2>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
         case 3: gstage<3>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
@@ -133,10 +135,11 @@ __device__ __forceinline__ int gntiles(const GenDev& G, int B)
 // Length-n transform of the nb pencils in `a` (the pre-chirp applied when
 // Bluestein), `t` scratch; returns the buffer holding the result (before
 // the Bluestein post-chirp).
-__device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const float2* __restrict__ tw, int nb,
-                              int sign)
+__device__ __forceinline__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl,
+                                              const float2* __restrict__ tw, const float2* __restrict__ stw, int nb,
+                                              int sign)
 {
-    float2* r = gfft(a, t, pl, tw, nb, pl.bluestein ? -1 : sign);
+    float2* r = gfft(a, t, pl, stw, nb, pl.bluestein ? -1 : sign);
     if (pl.bluestein) {
         const float2* H = tw + (sign < 0 ? pl.hf : pl.hi);
         const uint32_t mm = fdiv_magic((uint32_t)pl.m);
@@ -145,7 +148,7 @@ __device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const 
             r[idx] = gmul(r[idx], H[k]);
         }
         __syncthreads();
-        r = gfft(r, r == a ? t : a, pl, tw, nb, +1);
+        r = gfft(r, r == a ? t : a, pl, stw, nb, +1);
     }
     return r;
 }
@@ -155,16 +158,24 @@ __device__ float2* gtransform(float2* a, float2* t, const GenPlanDev& pl, const 
 // times K (JUNC 1, fft_mul_inplace, fft.c:437-453) or conj(K) (JUNC 2,
 // fft_mul_conj_inplace, fft.c:490-512), inverse transform, in LDS: the
 // junction of a convolution in one pass instead of three.
-template <int AXIS, int JUNC>
-__global__ __launch_bounds__(GATHR_MAX) void gk_axis(const GenDev* __restrict__ gd, int sign)
+template <int AXIS, int JUNC, bool TWL>
+__device__ __forceinline__ void gaxis_body(const GenDev& G, int sign)
 {
     extern __shared__ float2 sh[];
-    const GenDev& G = *gd;
     const GenPlanDev& pl = G.plan[AXIS];
     const int n = pl.n, m = pl.m, B = pl.B;
     float2* a = sh;
     float2* t = sh + (size_t)B * m;
     const float2* __restrict__ tw = G.tw;
+    // the stage twiddles: copied to LDS after the two pencil buffers (TWL),
+    // or read from memory
+    const float2* stw = tw + pl.twoff[0];
+    if (TWL) {
+        float2* twl = sh + 2 * (size_t)B * m;
+        for (int i = threadIdx.x; i < pl.twlen; i += GATHR) twl[i] = tw[pl.twoff[0] + i];
+        __syncthreads();
+        stw = twl;
+    }
     const float2* chirp = tw + pl.chirp;
     const int s_in = JUNC ? -1 : sign, s_out = JUNC ? +1 : sign;
     float2* __restrict__ Z = G.Z;
@@ -198,7 +209,7 @@ __global__ __launch_bounds__(GATHR_MAX) void gk_axis(const GenDev* __restrict__ 
             a[b * m + i] = v;
         }
         __syncthreads();
-        float2* r = gtransform(a, t, pl, tw, nb, s_in);
+        float2* r = gtransform(a, t, pl, tw, stw, nb, s_in);
         if (JUNC) {
             // spectrum (post-chirp) times K, then the inverse's pre-chirp,
             // into the other buffer (zero beyond n, as the load step pads)
@@ -218,7 +229,7 @@ __global__ __launch_bounds__(GATHR_MAX) void gk_axis(const GenDev* __restrict__ 
                 o[b * m + i] = v;
             }
             __syncthreads();
-            r = gtransform(o, o == a ? t : a, pl, tw, nb, s_out);
+            r = gtransform(o, o == a ? t : a, pl, tw, stw, nb, s_out);
         }
         for (int idx = threadIdx.x; idx < B * n; idx += GATHR) {
             const int q = (int)fdiv((uint32_t)idx, AXIS == 0 ? nmag : bmag);
@@ -231,6 +242,15 @@ __global__ __launch_bounds__(GATHR_MAX) void gk_axis(const GenDev* __restrict__ 
         }
         __syncthreads();
     }
+}
+
+template <int AXIS, int JUNC>
+__global__ __launch_bounds__(GATHR_MAX) void gk_axis(const GenDev* __restrict__ gd, int sign)
+{
+    if (gd->plan[AXIS].twlds)
+        gaxis_body<AXIS, JUNC, true>(*gd, sign);
+    else
+        gaxis_body<AXIS, JUNC, false>(*gd, sign);
 }
 
 __device__ __forceinline__ size_t gvol(const GenDev& G) { return (size_t)G.nx * G.ny * G.nz; }
