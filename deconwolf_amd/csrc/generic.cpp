@@ -38,6 +38,9 @@ enum { OP_PSF = 0, OP_OTF, OP_ONES, OP_WEIGHTS, OP_MULK, OP_MULKC, OP_UPDATE, OP
 #ifndef DWHIP_GEN_WG_LDS_KB
 #define DWHIP_GEN_WG_LDS_KB 150
 #endif
+#ifndef DWHIP_GEN_VALS
+#define DWHIP_GEN_VALS 6
+#endif
 constexpr size_t kGenLds = 150 * 1024;
 constexpr size_t kGenWgLds = DWHIP_GEN_WG_LDS_KB * 1024;
 constexpr int kElemGrid = 4096;    // grid of the elementwise kernels
@@ -239,9 +242,10 @@ int axis(const DwhipJob* j, int a, int sign, hipStream_t s)
 {
     const int ax = std::min(a, 2);
     const int grid = std::min(ntiles_axis(j->g, ax), 65536);
-    // threads for about eight values each of the tile's B*m (at least 256)
+    // threads for about DWHIP_GEN_VALS values each of the tile's B*m (256
+    // to 1024; a build tunable)
     const int vals = j->g.plan[ax].B * j->g.plan[ax].m;
-    const int threads = vals >= 8 * 1024 ? 1024 : vals >= 8 * 512 ? 512 : 256;
+    const int threads = vals >= DWHIP_GEN_VALS * 1024 ? 1024 : vals >= DWHIP_GEN_VALS * 512 ? 512 : 256;
     const hipError_t e = dwhip_gen_launch_axis(j->gd, a, sign, grid, dwhip_generic_lds(j, ax), threads, s);
     return e == hipSuccess ? DWHIP_OK : dwhip_hip_fail(e, "any-size axis pass", __LINE__);
 }
