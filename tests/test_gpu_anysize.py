@@ -102,3 +102,41 @@ def test_exact_size_in_engine_equals_solo():
             np.testing.assert_array_equal(st["err_trace"], s2.err_trace)
     finally:
         eng.close()
+
+
+def test_rel_any_size_and_fused_jobs_in_engine():
+    """REL jobs of both kinds in one engine: the any-size job's stopping
+    rule runs on the host (one round per step while it is active), the
+    fused-pass jobs' on the device; every job stops after its solo run's
+    iterations with a bit-identical result, and a fused job left alone
+    finishes in multi-round steps."""
+    print("any-size: REL jobs of both kinds in one engine", flush=True)
+    specs = [((12, 26, 37), (7, 9, 9), 1, 0.02, 12), ((10, 24, 30), (5, 7, 7), 2, 0.01, 40),
+             ((8, 20, 24), (5, 5, 5), 2, 1e-9, 30)]
+    jobs = []
+    for i, (sh, ps, bq, rel, mx) in enumerate(specs):
+        im, k = _case(40 + i, sh, ps)
+        s = dw.DwOpts(iter_type=dw.ITER_REL, err_rel=rel, maxiter=mx, borderQuality=bq)
+        jobs.append((torch.from_numpy(im).to(DEV), torch.from_numpy(k).to(DEV), s))
+    eng = Engine(hbm_budget_gb=1.0, max_active=8, rounds_per_step=8)
+    try:
+        for t, (im, k, s) in enumerate(jobs):
+            eng.submit(t, im, k, s)
+        done, steps = [], 0
+        while len(done) < len(jobs):
+            done += eng.step()
+            steps += 1
+            assert steps < 200
+        rounds = 0
+        for t, (im, k, s) in enumerate(jobs):
+            import copy
+            s2 = copy.copy(s)
+            ref = dw.deconvolve_shb_hip_device(im, k, s2)
+            out, st = eng.result(t)
+            assert st["iterations"] == s2.stats["iterations"], (t, st["iterations"], s2.stats["iterations"])
+            assert torch.equal(out, ref), t
+            rounds = max(rounds, st["iterations"])
+        # the last job runs to its maxiter of 30, past the any-size job's end
+        assert rounds == 30 and steps < rounds
+    finally:
+        eng.close()
