@@ -454,7 +454,7 @@ def main():
         print(json.dumps({"metric": METRIC, "sweep": rows, "engine_groups": ENGINE_GROUPS}))
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (jobs would shard one per GPU "
+        print("bench.py measures one GPU: libdwhip has no multi-GPU path (jobs would shard one per GPU "
               "with no communication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -51,7 +51,7 @@ int dwhip_init(void)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return DWHIP_ERR_NODEVICE;
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-        fprintf(stderr, "dwhip: device is %s, this build targets gfx950\n", prop.gcnArchName);
+        fprintf(stderr, "dwhip: device is %s, libdwhip is compiled for gfx950\n", prop.gcnArchName);
         return DWHIP_ERR_NODEVICE;
     }
     g_init = 1;
