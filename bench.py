@@ -15,9 +15,10 @@ Prints one JSON line.  `value` is image voxel-iterations per second
 `roofline` is for the kernel with the largest total time: its algorithmic
 bytes per launch over its mean launch duration, both from HIP events
 around every launch on the stream it runs on.  After the timed steps come
-the c4 request stream (`served`), the c3 tiled image (`tiled`) and a c5
+the c4 request stream (`served`), the c3 tiled image (`tiled`), a c5
 sample (`c5`: the four channels of one multi-channel field, engine against
-plain), each with its own rates; none of them is `value`.
+plain) and the c2 image at border quality 1 on the any-size path
+(`anysize`), each with its own rates; none of them is `value`.
 """
 import argparse
 import copy
@@ -394,6 +395,7 @@ def main():
                          "engine) and plain (one at a time); 0 skips it")
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate of the stream, jobs/s")
     ap.add_argument("--no-tiled", action="store_true", help="skip the c3 tiled run")
+    ap.add_argument("--no-anysize", action="store_true", help="skip the c2 border-quality-1 (any-size path) timing")
     ap.add_argument("--c5-fields", type=int, default=1, help="fields of the c5 sample (4 channels each); 0 skips it")
     ap.add_argument("--engine-groups", type=int, default=ENGINE_GROUPS, choices=(1, 2),
                     help="job groups of the engine runs (2: two streams whose passes overlap)")
@@ -528,6 +530,7 @@ def main():
         served = run_stream(a.stream, a.rate)
     tiled = None if a.no_tiled else run_tiled()
     c5 = run_c5(a.c5_fields) if a.c5_fields > 0 else None
+    anysize = None if a.no_anysize else run_anysize()
 
     res = {
         "metric": METRIC,
@@ -574,6 +577,7 @@ def main():
         "served": served,
         "tiled": tiled,
         "c5": c5,
+        "anysize": anysize,
         "engine_groups": ENGINE_GROUPS,
         "kernels": kernels,
         "input_gen_s": t_gen,
