@@ -335,9 +335,10 @@ size_t dwhip_job_bytes(const int64_t* work, int64_t M, int64_t N, int64_t P, int
                        int64_t pP, int border_quality, int niter)
 {
     if (!dwhip_fast_size_ok(work)) {
-        // any-size path: complex Z and K, U0, U1 (and W), partials, tables
+        // any-size path: half spectra Z and K, U0, U1 (and W), partials, tables
         const size_t V = (size_t)work[0] * work[1] * work[2];
-        size_t b = 2 * V * sizeof(float2) + (border_quality > 0 ? 3 : 2) * V * sizeof(float);
+        const size_t Vh = round_up(work[0] / 2 + 1, 16) * work[1] * work[2];
+        size_t b = 2 * Vh * sizeof(float2) + (border_quality > 0 ? 3 : 2) * V * sizeof(float);
         b += (size_t)dwhip_generic_ratio_grid() * sizeof(double);
         b += (size_t)(niter + 1) * (sizeof(float) + sizeof(double));
         b += DWHIP_STAT_BLOCKS * 4 * sizeof(double) + 4096 + sizeof(GenDev) + sizeof(JobDev);
@@ -519,8 +520,9 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
     d.start_img = start_dev;
 
     if (j->generic) {
-        // any-size path: complex volumes Z (in C) and K of nx*ny*nz values;
-        // k_round_end adds up the ratio kernel's one partial per workgroup
+        // any-size path: half spectra Z (in C) and K of hcs*ny*nz values, Z
+        // also holding real rows of 2*hcs floats; k_round_end adds up the
+        // ratio kernel's one partial per workgroup
         const int grc = dwhip_generic_plan(j);
         if (grc != DWHIP_OK) return grc;
         j->rtw_off = 0;
@@ -530,8 +532,9 @@ int dwhip_job_plan(DwhipJob* j, const float* im_dev, int64_t M, int64_t N, int64
         size_t off = 0;
         auto take = [&](size_t bytes) { size_t o = off; off = round_up(off + bytes, 256); return o; };
         DwhipLayout& L = j->lay;
-        L.C = take(V * sizeof(float2));
-        L.K = take(V * sizeof(float2));
+        const size_t Vh = (size_t)d.hcs * d.ny * d.nz;
+        L.C = take(Vh * sizeof(float2));
+        L.K = take(Vh * sizeof(float2));
         L.U0 = take(V * sizeof(float));
         L.U1 = take(V * sizeof(float));
         L.W = d.has_w ? take(V * sizeof(float)) : 0;

@@ -49,8 +49,8 @@ struct DwhipJob {
     size_t rtw_off = 0;
     BatchDev bh;              // host copies of the one-job table being uploaded
     JobDev dcopy;
-    // any-size path (generic.cpp): full complex volumes, per-axis FFTs of
-    // any length; C holds the complex work volume Z, K the OTF
+    // any-size path (generic.cpp): per-axis FFTs of any length; C holds
+    // the work volume Z (half spectra, or real rows in place), K the OTF
     bool generic = false;
     GenDev g;                 // host copy
     GenDev* gd = nullptr;     // device copy (in the job block)

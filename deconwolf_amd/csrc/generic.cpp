@@ -129,6 +129,7 @@ int dwhip_generic_plan(DwhipJob* j)
     memset(&g, 0, sizeof(g));
     const JobDev& d = j->d;
     g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
+    g.hc = d.hc; g.hs = d.hcs;
     g.M = d.M; g.N = d.N; g.P = d.P;
     g.pM = d.pM; g.pN = d.pN; g.pP = d.pP;
     g.rl = d.rl; g.positivity = d.positivity; g.metric = d.metric; g.has_w = d.has_w;
@@ -229,11 +230,13 @@ int dwhip_generic_bind(DwhipJob* j, char* gdev, hipStream_t s)
 
 namespace {
 
+// tiles of an axis pass: B row pairs along x, B bins of a plane (y) or a
+// row (z)
 int ntiles_axis(const GenDev& g, int a)
 {
     const int B = g.plan[a].B;
-    if (a == 0) return (g.ny * g.nz + B - 1) / B;
-    const int nxb = (g.nx + B - 1) / B;
+    if (a == 0) return ((g.ny * g.nz + 1) / 2 + B - 1) / B;
+    const int nxb = (g.hc + B - 1) / B;
     return nxb * (a == 1 ? g.nz : g.ny);
 }
 

@@ -1,10 +1,13 @@
-// Device descriptors of the any-size path: deconwolf's SHB and RL loops on
-// full complex volumes, for work sizes the fused passes cannot transform
-// (border quality 0 and 1 at deconwolf's exact size, whose extents may hold
-// any prime factor or be odd).  Each axis is a complex FFT of its length n:
-// Stockham passes in LDS with radices 2, 3, 4, 5 and 7 when n factors into
-// them, and Bluestein's chirp-z form otherwise (a length-m FFT pair with
-// m >= 2n - 1 that does factor).
+// Device descriptors of the any-size path: deconwolf's SHB and RL loops for
+// work sizes the fused passes cannot transform (border quality 0 and 1 at
+// deconwolf's exact size, whose extents may hold any prime factor or be
+// odd).  Each axis is a complex FFT of its length n: Stockham passes in LDS
+// with radices 2, 3, 4, 5 and 7 when n factors into them, and Bluestein's
+// chirp-z form otherwise (a length-m FFT pair with m >= 2n - 1 that does
+// factor).  The volumes are real, so x is a real transform of any length:
+// two real rows as the real and imaginary parts of one complex row, their
+// half spectra (nx/2 + 1 bins) separated after the forward transform and
+// recombined before the inverse; y and z run over those bins only.
 #pragma once
 #include <stdint.h>
 
@@ -29,12 +32,13 @@ struct GenPlanDev {
 
 struct GenDev {
     int nx, ny, nz;
+    int hc, hs;       // x bins kept (nx/2 + 1) and the complex row stride of Z and K (>= hc)
     int M, N, P;
     int pM, pN, pP;
     int rl, positivity, metric, has_w;
     float alphamax;
-    float2* Z;        // complex work volume [nz][ny][nx]
-    float2* K;        // OTF / (nx*ny*nz), [nz][ny][nx]
+    float2* Z;        // half spectra [nz][ny][hs] (bins < hc), or real rows of 2*hs floats in place
+    float2* K;        // OTF / (nx*ny*nz), half spectrum [nz][ny][hs]
     float* U0;        // x_k lives in U[k & 1], nx*ny*nz
     float* U1;
     float* W;         // Bertero weights (has_w)
