@@ -32,14 +32,15 @@ enum { OP_PSF = 0, OP_OTF, OP_ONES, OP_WEIGHTS, OP_MULK, OP_MULKC, OP_UPDATE, OP
 
 // LDS for the two ping-pong buffers of an axis workgroup: at most kGenLds,
 // and pencils per workgroup chosen within kGenWgLds (a build tunable, KiB).
-// The axis kernels are bound by latency more than by bytes: long pencils
-// (Bluestein lengths) run one workgroup of 1024 threads per CU with the
-// whole budget, short ones several workgroups of 256 threads.
+// The axis kernels are bound by latency more than by bytes: a long pencil
+// (a Bluestein length, 2240 for n = 1115) runs alone in a workgroup of 512
+// threads, two workgroups per CU (54 KiB each with the stage twiddles),
+// short ones 16 to a workgroup of 256 threads, several per CU.
 #ifndef DWHIP_GEN_WG_LDS_KB
-#define DWHIP_GEN_WG_LDS_KB 150
+#define DWHIP_GEN_WG_LDS_KB 75
 #endif
 #ifndef DWHIP_GEN_VALS
-#define DWHIP_GEN_VALS 6
+#define DWHIP_GEN_VALS 4
 #endif
 constexpr size_t kGenLds = 150 * 1024;
 constexpr size_t kGenWgLds = DWHIP_GEN_WG_LDS_KB * 1024;
