@@ -55,10 +55,15 @@ bool smooth2357(int64_t n)
     return n == 1;
 }
 
-// radices of a 2,3,5,7-smooth m: fours first, then 2, 3, 5, 7
+// radices of a 2,3,5,7-smooth m: eights first (DWHIP_GEN_R8), then fours,
+// then 2, 3, 5, 7
 std::vector<int> gen_radices(int m)
 {
     std::vector<int> r;
+    while (DWHIP_GEN_R8 && m % 8 == 0) {
+        r.push_back(8);
+        m /= 8;
+    }
     while (m % 4 == 0) {
         r.push_back(4);
         m /= 4;

@@ -14,6 +14,10 @@
 #include "shb_dev.h"
 
 #define GEN_MAX_STAGES 16
+// radix-8 stages in the any-size FFTs (a build tunable)
+#ifndef DWHIP_GEN_R8
+#define DWHIP_GEN_R8 0
+#endif
 
 struct GenPlanDev {
     int n;           // transform length along the axis

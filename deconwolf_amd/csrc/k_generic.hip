@@ -68,9 +68,9 @@ __device__ __forceinline__ void gstage(const float2* __restrict__ src, float2* _
         }
         // the fused passes' small DFTs (fft_engine.h)
         if (sign < 0)
-            Dft<R, -1>::run(v);
+            DftC<R, -1>::run(v);
         else
-            Dft<R, +1>::run(v);
+            DftC<R, +1>::run(v);
         const int o = b * m + (jj - k) * R + k;
 #pragma unroll
         for (int t = 0; t < R; ++t) dst[o + t * ns] = v[t];
@@ -91,6 +91,9 @@ __device__ __forceinline__ float2* gfft(float2* a, float2* t, const GenPlanDev& 
         case 3: gstage<3>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
         case 4: gstage<4>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
         case 5: gstage<5>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
+#if DWHIP_GEN_R8
+        case 8: gstage<8>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
+#endif
         default: gstage<7>(src, dst, pl.m, nb, pl.ns[s], tws, sign); break;
         }
         __syncthreads();
