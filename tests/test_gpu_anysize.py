@@ -2,7 +2,8 @@
 exact work size (method_shb.c:137-153) when that size has an odd x extent
 or a prime factor above 7, which the fused passes cannot transform.  Each
 axis is a complex FFT in LDS (Stockham, or Bluestein's chirp-z form for
-other factors).  Compared with the float32 oracle at the same size (its own
+other factors; x as a real transform of two rows per pencil, y and z over
+its half spectrum).  Compared with the float32 oracle at the same size (its own
 transforms, Bluestein above 61): relative L2 <= 1e-5 and error traces to
 1e-4, SHB and RL, FIXED and REL; in the engine, bit-identical to solo
 runs beside batched jobs.
@@ -42,6 +43,10 @@ CASES = [
     ((12, 26, 37), (7, 9, 9), 1),     # 42 x 31 x 16: 31 prime
     ((9, 40, 52), (7, 11, 11), 1),    # 58 x 46 x 13: 29, 23, 13
     ((20, 61, 64), (9, 9, 9), 0),     # 64 x 61 x 20: 61 prime
+    # an odd number of x-rows (ny * nz): the last row pair of the real x
+    # transform holds one row
+    ((9, 27, 35), (7, 9, 9), 0),      # 35 x 27 x 9: odd x, no Bluestein, 243 rows
+    ((11, 23, 37), (7, 9, 9), 0),     # 37 x 23 x 11: primes, 253 rows
 ]
 
 
