@@ -36,7 +36,22 @@ __global__ __launch_bounds__(STHR) void k_stats_partial(const JobDev* __restrict
     const size_t n = (size_t)J.M * J.N * J.P;
     const size_t np = (size_t)J.pM * J.pN * J.pP;
     double vmin = INFINITY, vsum = 0, vmax = -INFINITY, vidx = 1e300;
-    for (size_t i = (size_t)blockIdx.x * STHR + threadIdx.x; i < n; i += (size_t)nblk * STHR) {
+    // each thread visits i, i + S, i + 2S, ... in that order; eight loads
+    // are issued before they are added (the same sum, more loads in flight)
+    const size_t S = (size_t)nblk * STHR;
+    size_t i = (size_t)blockIdx.x * STHR + threadIdx.x;
+    for (; i + 7 * S < n; i += 8 * S) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = J.im[i + q * S];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double d = v[q];
+            vmin = d < vmin ? d : vmin;
+            vsum += d;
+        }
+    }
+    for (; i < n; i += S) {
         const double v = J.im[i];
         vmin = v < vmin ? v : vmin;
         vsum += v;
@@ -181,12 +196,13 @@ __global__ __launch_bounds__(STHR) void k_crop(const JobDev* __restrict__ jobs, 
 {
     const JobDev& J = jobs[j];
     const float* U = (k & 1) ? J.U1 : J.U0;
-    const size_t n = (size_t)J.M * J.N * J.P;
-    for (size_t i = (size_t)blockIdx.x * STHR + threadIdx.x; i < n; i += (size_t)gridDim.x * STHR) {
-        const int m = (int)(i % J.M);
-        const size_t r = i / J.M;
-        const int yy = (int)(r % J.N), zz = (int)(r / J.N);
-        out[i] = U[m + (size_t)J.nx * (yy + (size_t)J.ny * zz)];
+    // by image rows: workgroups over the rows, threads along a row
+    const int nrows = J.N * J.P;
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int zz = r / J.N, yy = r - zz * J.N;
+        const float* src = U + (size_t)J.nx * (yy + (size_t)J.ny * zz);
+        float* dst = out + (size_t)J.M * r;
+        for (int m = threadIdx.x; m < J.M; m += STHR) dst[m] = src[m];
     }
 }
 
