@@ -119,6 +119,7 @@ def _against_oracles(config, im, psf, niter, threads):
     float32 computation is (two independent float32 trajectories are
     about sqrt(2) d apart)."""
     out, s, t_gpu = _gpu(im, psf, niter)
+    print("%s: GPU run done in %.1f s" % (config, t_gpu), flush=True)
     ref = tuple(s.stats["ref_work"])
     res = {}
     for dt in (np.float32, np.float64):
@@ -126,6 +127,7 @@ def _against_oracles(config, im, psf, niter, threads):
         o, e, bg, n = po.shb(im, psf, niter=niter, work=ref, nthreads=threads, dtype=dt)
         assert n == niter and s.bg == np.float32(bg)
         res[dt] = (o, e, time.time() - t)
+        print("%s: %s oracle done in %.1f s" % (config, np.dtype(dt).name, res[dt][2]), flush=True)
     d = rel_l2(res[np.float32][0], res[np.float64][0])
     bar = max(TOL, 2 * d)
     for dt, (o, e, t_cpu) in res.items():
