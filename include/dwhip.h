@@ -128,8 +128,10 @@ int dwhip_init(void);
  * engine transforms: for border_quality 2 the smallest size >= ref_work
  * with an even x extent whose halves and the y, z extents factor into 2, 3,
  * 5 and 7 (the image region is unchanged by that padding); for quality 0
- * and 1 the reference size itself, which must already factor that way.
- * Returns DWHIP_OK or DWHIP_ERR_SIZE.
+ * and 1 the reference size itself, run by the fused passes when it factors
+ * that way, else by the any-size path (odd extents and any prime factor,
+ * through Bluestein's form; extents up to about 4800).
+ * Returns DWHIP_OK or DWHIP_ERR_SIZE (an extent beyond both).
  */
 int dwhip_work_size(int64_t M, int64_t N, int64_t P, int64_t pM, int64_t pN, int64_t pP,
                     int border_quality, int64_t *work, int64_t *ref_work);
