@@ -262,6 +262,13 @@ __host__ __device__ constexpr bool x_stage_rows(int mode)
 #define DWHIP_NT_COLS 1
 #endif
 __host__ __device__ constexpr bool x_nt(int mode) { return mode == XM_UPDATE ? DWHIP_NT_UPDATE : DWHIP_NT_X; }
+// the update's state streams (x_{k-1}, x_{k-2} in and x_k out, W), read or
+// written once, hinted apart from its spectrum reads: they then leave more
+// of L2 to the spectrum lines its split reads twice and to the spectrum it
+// writes, which the next pass (y) reads
+#ifndef DWHIP_NT_STATE
+#define DWHIP_NT_STATE 1
+#endif
 
 // Element loops of the two iteration passes specialised for their common
 // case (SHB iteration k > 0 of the update pass; the SHB ratio pass, whose
@@ -609,9 +616,9 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
                     const int w = w0 + q * XTHR;
                     if (!FULL && w >= nwork) continue;
                     a[q] = lds[w];
-                    b[q] = ldgs<XNT>(xm1 + w);
-                    c[q] = ldgs<XNT>(xio + w);
-                    wv[q] = HW ? ldgs<XNT>(wt + w) : make_float2(1.f, 1.f);
+                    b[q] = ldgs<DWHIP_NT_STATE>(xm1 + w);
+                    c[q] = ldgs<DWHIP_NT_STATE>(xio + w);
+                    wv[q] = HW ? ldgs<DWHIP_NT_STATE>(wt + w) : make_float2(1.f, 1.f);
                 }
 #pragma unroll
                 for (int q = 0; q < XCHUNK; ++q) {
@@ -624,7 +631,7 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
                                    : make_float2(a[q].x * p0, a[q].y * p1);
                     xk.x = xk.x < lo ? lo : xk.x;
                     xk.y = xk.y < lo ? lo : xk.y;
-                    stgs<XNT>(xio + w, xk);
+                    stgs<DWHIP_NT_STATE>(xio + w, xk);
                     lds[w] = make_float2(shb_extrap(xk.x, b[q].x, al, bg), shb_extrap(xk.y, b[q].y, al, bg));
                 }
             };
