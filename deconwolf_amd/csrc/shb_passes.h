@@ -782,6 +782,24 @@ __device__ __forceinline__ void xpass_tile(const JobDev& J, int t, int tid, floa
     }
 }
 
+// Tile order of a pass: the update and ratio passes can walk their tiles
+// from the last plane to the first (DWHIP_REV_UPDATE, DWHIP_REV_RATIO), so
+// that they start on the planes the pass before them wrote last and end on
+// the planes the pass after them reads first, which the device's last-level
+// cache may still hold.  The update pass does (its y neighbours run from
+// the first plane to the last).
+#ifndef DWHIP_REV_UPDATE
+#define DWHIP_REV_UPDATE 1
+#endif
+#ifndef DWHIP_REV_RATIO
+#define DWHIP_REV_RATIO 0
+#endif
+template <int MODE>
+constexpr bool x_reversed()
+{
+    return (MODE == XM_UPDATE && DWHIP_REV_UPDATE) || (MODE == XM_RATIO && DWHIP_REV_RATIO);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __restrict__ bd, const JobDev* __restrict__ jobs)
 {
@@ -790,7 +808,8 @@ __global__ __launch_bounds__(XTHR) DWHIP_X_OCC void k_xpass(const BatchDev* __re
     constexpr int kind = xmode_kind(MODE);
     const int total = bd->ntile[kind];
     const int njobs = bd->njobs;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    for (int t0 = blockIdx.x; t0 < total; t0 += gridDim.x) {
+        const int t = x_reversed<MODE>() ? total - 1 - t0 : t0;
         const JobDev& J = jobs[find_job(jobs, njobs, kind, t)];
         if (J.dev_stop && gp(J.st)->stop) continue;  // the stopping rule ended this job
         if constexpr (x_fixed_mode(MODE)) {
